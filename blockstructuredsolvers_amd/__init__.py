@@ -1,0 +1,11 @@
+"""blockstructuredsolvers_amd -- MI355X-native recursive Schur-complement
+block-tridiagonal Cholesky solver (the batched GPU path of BlockStructuredSolvers).
+
+The compute path is libbss.so (hand-written HIP kernels for gfx950, C ABI in
+include/bss.h); this package is the thin host mirror of the reference API.
+"""
+from .solver import (BlockTriDiagData_batched, factorize, factorize_, initialize_batched,
+                     reference_schedule, solve, solve_)
+
+__all__ = ["BlockTriDiagData_batched", "initialize_batched", "factorize", "solve",
+           "factorize_", "solve_", "reference_schedule"]

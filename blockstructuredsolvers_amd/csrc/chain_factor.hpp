@@ -1,0 +1,364 @@
+// chain_factor.hpp -- fused per-chain factorisation kernel of one recursion level.
+//
+// A level holds N diagonal blocks A_i and N-1 couplings B_i (M = tridiag(B^T, A, B)).
+// The schedule splits it into chains of consecutive interior blocks separated by
+// single separator blocks.  For a chain [s .. s+m-1] with left separator L = s-1 and
+// right separator R = s+m (either may be absent) the kernel eliminates the chain in
+// order, keeping the fill that the left separator creates ("panel" Z):
+//
+//   Z_s = B_L^T,  for i = s .. s+m-1:
+//     A_i  <- A_i - C_{i-1}^T C_{i-1}               (Schur update, MFMA)
+//     U_i^T U_i = A_i ; Uinv_i = U_i^{-1}           (blocked potrf + inverse)
+//     [C_i | W_i] = Uinv_i^T [B_i | Z_i]            (trsm as MFMA products)
+//     Z_{i+1} = -C_i^T W_i ;  S_L += W_i^T W_i      (MFMA)
+//   reduced system: A'_L -= S_L, A'_R -= C_last^T C_last, B'_{L,R} = -W_last^T C_last
+//
+// This is the reference's per-level work -- cholesky_factorize_batched! (factor the
+// P-1 interior chains, src/backbone_rocm.jl:62-82), cholesky_solve_batched! with the
+// 2n-column G (src/backbone_rocm.jl:84-120) and compute_schur_complement!
+// (src/backbone_rocm.jl:122-138) -- re-derived so that only the one-sided panel
+// W = U^{-T} G is formed (G is non-zero only in the first and last interior block,
+// src/gpu_batched.jl:214-215) and the Schur complement G^T T^{-1} G = W^T W is
+// accumulated as the chain is eliminated, in one launch per level with the blocks
+// held in LDS.
+//
+// Stored factors (consumed by the substitution sweeps): Uinv_i (upper 16x16 tiles of
+// the A slot), C_i (the B slot), W_i (panel slot, only for chains with a left
+// separator).  One workgroup per chain; blockIdx.y is the system in a batch.
+#pragma once
+
+#include "common.hpp"
+
+namespace bss {
+
+template <typename T>
+struct FactorLevelArgs {
+  const void* A_in;  // diagonal blocks (input element type), N per system
+  const void* B_in;  // couplings (input element type), N-1 per system
+  T* U_out;          // Uinv, N per system (may alias A_in)
+  T* C_out;          // C, N-1 per system (may alias B_in)
+  T* W_out;          // panel W, N per system
+  T* SR;             // P per system: C_last^T C_last of the chain left of separator p
+  T* SL;             // P per system: sum W^T W of the chain right of separator p
+  T* Bnext;          // P-1 per system: reduced coupling separator p -> p+1
+  const int4* chains;  // {start, len, sepL, sepR}, sep = -1 when absent
+  const int* orig;     // N: level-0 block index of each block (info reporting)
+  int* info;           // per system, atomicMin of 1-based failing block index
+  long long sysA, sysB, sysU, sysC, sysW, sysS, sysBn;  // per-system element strides
+};
+
+template <typename T>
+struct LdsPad {
+  static constexpr int value = sizeof(T) == 8 ? 2 : 4;  // 16-byte column shift
+};
+
+__device__ __host__ constexpr int tile_off(int r, int c, int ld) { return 16 * c * ld + 16 * r; }
+
+// Wave-level Cholesky + inverse of one 16x16 SPD tile held (upper triangle) in LDS.
+// Lane l owns column l%16, rows 4*(l/16)..+3.  Row operations are applied to
+// [A | I]; at the end the right half is U^{-T}.  Writes Uinv = (U^{-T})^T into the
+// upper triangle of the output tile (strict lower part exact zeros).  Returns false
+// if a pivot was not positive.
+template <typename T>
+__device__ inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+  T a[4], e[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 4 * lk + v;
+    a[v] = (r <= lr) ? at[lr * lda + r] : at[r * lda + lr];
+    e[v] = (r == lr) ? T(1) : T(0);
+  }
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int sg = j >> 2, sv = j & 3;
+    const T p = __shfl(a[sv], j + 16 * sg);
+    ok = ok && (p > T(0));
+    const T s = rsqrt_t<T>(p);
+    const T ujc = __shfl(a[sv], lr + 16 * sg) * s;
+    const T ejc = __shfl(e[sv], lr + 16 * sg) * s;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * lk + v;
+      const T ujr = __shfl(a[sv], r + 16 * sg) * s;
+      if (r > j) {
+        a[v] -= ujr * ujc;
+        e[v] -= ujr * ejc;
+      } else if (r == j) {
+        a[v] = ujc;
+        e[v] = ejc;
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) ut[(4 * lk + v) * ldu + lr] = e[v];
+  return ok;
+}
+
+// Blocked potrf + inverse of the NB x NB block in Abuf (upper triangle used) with
+// 16x16 tiles: right-looking elimination on [A | I] where the diagonal tiles go
+// through diag_chol_inv16 and every panel/trailing/inverse update is an MFMA tile
+// product.  On exit Ubuf's upper tiles hold Uinv = U^{-1}.  Abuf is consumed.
+template <typename T, int NB, int LD, int NW>
+__device__ inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
+  constexpr int TT = NB / 16;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int w = wave_id();
+  for (int kk = 0; kk < TT; ++kk) {
+    if (w == 0) {
+      const bool ok = diag_chol_inv16<T>(Abuf + tile_off(kk, kk, LD), LD,
+                                         Ubuf + tile_off(kk, kk, LD), LD);
+      if (!ok && lane_id() == 0) *fail = 1;
+    }
+    __syncthreads();
+    const int nrow = TT - 1 - kk;
+    // scaling: U_{kk,j} = Uinv_kk^T A_{kk,j} (j > kk);  Et_{t,kk} <- Et_{t,kk} Uinv_kk (t < kk)
+    for (int task = w; task < TT - 1; task += NW) {
+      acc_t acc = Mfma<T>::zero();
+      if (task < nrow) {
+        const int j = kk + 1 + task;
+        tile_mma<T, true, false, false>(acc, Ubuf + tile_off(kk, kk, LD), LD,
+                                        Abuf + tile_off(kk, j, LD), LD, 16);
+        tile_store<T>(Abuf + tile_off(kk, j, LD), LD, acc);
+      } else {
+        const int t = task - nrow;
+        tile_mma<T, false, false, false>(acc, Ubuf + tile_off(t, kk, LD), LD,
+                                         Ubuf + tile_off(kk, kk, LD), LD, 16);
+        tile_store<T>(Ubuf + tile_off(t, kk, LD), LD, acc);
+      }
+    }
+    __syncthreads();
+    if (nrow > 0) {
+      // elimination: A_ij -= U_{kk,i}^T U_{kk,j} (kk < i <= j);  Et_{t,i} -= Et_{t,kk} U_{kk,i}
+      const int nA = nrow * (nrow + 1) / 2;
+      const int nE = (kk + 1) * nrow;
+      for (int task = w; task < nA + nE; task += NW) {
+        if (task < nA) {
+          int idx = task, ii = 0;
+          while (idx >= nrow - ii) {
+            idx -= nrow - ii;
+            ++ii;
+          }
+          const int i = kk + 1 + ii, j = i + idx;
+          acc_t acc = tile_load<T>(Abuf + tile_off(i, j, LD), LD);
+          tile_mma<T, true, false, true>(acc, Abuf + tile_off(kk, i, LD), LD,
+                                         Abuf + tile_off(kk, j, LD), LD, 16);
+          tile_store<T>(Abuf + tile_off(i, j, LD), LD, acc);
+        } else {
+          const int e = task - nA;
+          const int t = e % (kk + 1), i = kk + 1 + e / (kk + 1);
+          acc_t acc = (t == kk) ? Mfma<T>::zero() : tile_load<T>(Ubuf + tile_off(t, i, LD), LD);
+          tile_mma<T, false, false, true>(acc, Ubuf + tile_off(t, kk, LD), LD,
+                                          Abuf + tile_off(kk, i, LD), LD, 16);
+          tile_store<T>(Ubuf + tile_off(t, i, LD), LD, acc);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// global (Tin) NB x NB block -> LDS (T), optionally transposed
+template <typename Tin, typename T, int NB, int LD, int NT, bool TRANS>
+__device__ inline void load_block(const Tin* __restrict__ g, T* s) {
+  for (int idx = threadIdx.x; idx < NB * NB; idx += NT) {
+    const int r = idx % NB, c = idx / NB;
+    const T v = (T)g[idx];
+    if (TRANS)
+      s[r * LD + c] = v;
+    else
+      s[c * LD + r] = v;
+  }
+}
+
+template <typename Tin, typename T, int NB, int NW>
+__global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T> p) {
+  constexpr int TT = NB / 16;
+  constexpr int LD = NB + LdsPad<T>::value;
+  constexpr int NT = 64 * NW;
+  constexpr int NN = NB * NB;
+  constexpr int NS = TT * (TT + 1) / 2;           // upper tiles of a symmetric block
+  constexpr int SQ = (NS + NW - 1) / NW;          // S accumulators per wave
+  constexpr int XQ = (2 * TT * TT + NW - 1) / NW; // trsm output tiles per wave
+  constexpr int ZQ = (TT * TT + NW - 1) / NW;     // panel tiles per wave
+  constexpr int PQ = (TT * TT + NS + NW - 1) / NW;
+  typedef typename Mfma<T>::acc_t acc_t;
+
+  __shared__ T Abuf[NB * LD];
+  __shared__ T Ubuf[NB * LD];
+  __shared__ T Rbuf[2 * NB * LD];  // columns [0,NB): B -> C ; [NB,2NB): Z -> W
+  __shared__ int fail;
+
+  const int4 cd = p.chains[blockIdx.x];
+  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  if (len <= 0) return;
+  const int sys = blockIdx.y;
+  const bool hasL = sepL >= 0, hasR = sepR >= 0;
+  const Tin* A = (const Tin*)p.A_in + sys * p.sysA;
+  const Tin* B = (const Tin*)p.B_in + sys * p.sysB;
+  T* U = p.U_out + sys * p.sysU;
+  T* C = p.C_out + sys * p.sysC;
+  T* W = p.W_out + sys * p.sysW;
+  const int w = wave_id();
+  const int tid = threadIdx.x;
+
+  acc_t Sacc[SQ];
+#pragma unroll
+  for (int q = 0; q < SQ; ++q) Sacc[q] = Mfma<T>::zero();
+
+  if (tid == 0) fail = 0;
+  load_block<Tin, T, NB, LD, NT, false>(A + (long long)start * NN, Abuf);
+  if (hasL) load_block<Tin, T, NB, LD, NT, true>(B + (long long)(start - 1) * NN, Rbuf + NB * LD);
+  __syncthreads();
+
+  for (int i = 0; i < len; ++i) {
+    const int blk = start + i;
+    const bool last = (i == len - 1);
+    const bool hasC = !last || hasR;  // block couples to a later block
+    if (hasC) load_block<Tin, T, NB, LD, NT, false>(B + (long long)blk * NN, Rbuf);
+
+    potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
+    if (tid == 0 && fail) {
+      const int ob = p.orig ? p.orig[blk] : blk;
+      atomicMin(p.info + sys, ob + 1);
+      fail = 0;
+    }
+
+    // [C | W] = Uinv^T [B | Z]
+    const int c0 = hasC ? 0 : TT;
+    const int c1 = hasL ? 2 * TT : TT;
+    const int ntile = TT * (c1 - c0);
+    acc_t X[XQ];
+#pragma unroll
+    for (int q = 0; q < XQ; ++q) {
+      const int task = w + NW * q;
+      X[q] = Mfma<T>::zero();
+      if (task < ntile) {
+        const int r = task % TT, c = c0 + task / TT;
+        for (int t = 0; t <= r; ++t)
+          tile_mma<T, true, false, false>(X[q], Ubuf + tile_off(t, r, LD), LD,
+                                          Rbuf + tile_off(t, c, LD), LD, 16);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < XQ; ++q) {
+      const int task = w + NW * q;
+      if (task < ntile) {
+        const int r = task % TT, c = c0 + task / TT;
+        tile_store<T>(Rbuf + tile_off(r, c, LD), LD, X[q]);
+        if (c < TT)
+          tile_store<T>(C + (long long)blk * NN + tile_off(r, c, NB), NB, X[q]);
+        else
+          tile_store<T>(W + (long long)blk * NN + tile_off(r, c - TT, NB), NB, X[q]);
+      }
+    }
+    for (int idx = tid; idx < NN; idx += NT) {
+      const int r = idx % NB, c = idx / NB;
+      if ((r >> 4) <= (c >> 4)) U[(long long)blk * NN + idx] = Ubuf[c * LD + r];
+    }
+    if (!last) load_block<Tin, T, NB, LD, NT, false>(A + (long long)(blk + 1) * NN, Abuf);
+    __syncthreads();
+
+    // S_L += W^T W (upper tiles, persistent per wave)
+    if (hasL) {
+#pragma unroll
+      for (int q = 0; q < SQ; ++q) {
+        int s = w + NW * q;
+        if (s < NS) {
+          int a = 0;
+          while (s >= TT - a) {
+            s -= TT - a;
+            ++a;
+          }
+          const int b = a + s;
+          tile_mma<T, true, false, false>(Sacc[q], Rbuf + (NB + 16 * a) * LD, LD,
+                                          Rbuf + (NB + 16 * b) * LD, LD, NB);
+        }
+      }
+    }
+    if (!last) {
+      // Z_{i+1} = -C^T W (held until all reads of Rbuf are done); A_{i+1} -= C^T C
+      acc_t Z[ZQ];
+#pragma unroll
+      for (int q = 0; q < PQ; ++q) {
+        const int task = w + NW * q;
+        if (task < TT * TT) {
+          if (hasL && q < ZQ) {
+            const int a = task % TT, b = task / TT;
+            Z[q] = Mfma<T>::zero();
+            tile_mma<T, true, false, true>(Z[q], Rbuf + 16 * a * LD, LD,
+                                           Rbuf + (NB + 16 * b) * LD, LD, NB);
+          }
+        } else if (task < TT * TT + NS) {
+          int s = task - TT * TT, a = 0;
+          while (s >= TT - a) {
+            s -= TT - a;
+            ++a;
+          }
+          const int b = a + s;
+          acc_t acc = tile_load<T>(Abuf + tile_off(a, b, LD), LD);
+          tile_mma<T, true, false, true>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD, NB);
+          tile_store<T>(Abuf + tile_off(a, b, LD), LD, acc);
+        }
+      }
+      __syncthreads();
+      if (hasL) {
+#pragma unroll
+        for (int q = 0; q < ZQ; ++q) {
+          const int task = w + NW * q;
+          if (task < TT * TT) {
+            const int a = task % TT, b = task / TT;
+            tile_store<T>(Rbuf + tile_off(a, TT + b, LD), LD, Z[q]);
+          }
+        }
+      }
+    } else {
+      // reduced-system contributions of the last block
+      if (hasR) {
+        T* sr = p.SR + sys * p.sysS + (long long)sepR * NN;
+        for (int task = w; task < NS; task += NW) {
+          int s = task, a = 0;
+          while (s >= TT - a) {
+            s -= TT - a;
+            ++a;
+          }
+          const int b = a + s;
+          acc_t acc = Mfma<T>::zero();
+          tile_mma<T, true, false, false>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD, NB);
+          tile_store<T>(sr + tile_off(a, b, NB), NB, acc);
+        }
+        if (hasL) {
+          T* bn = p.Bnext + sys * p.sysBn + (long long)sepL * NN;
+          for (int task = w; task < TT * TT; task += NW) {
+            const int a = task % TT, b = task / TT;
+            acc_t acc = Mfma<T>::zero();
+            tile_mma<T, true, false, true>(acc, Rbuf + (NB + 16 * a) * LD, LD,
+                                           Rbuf + 16 * b * LD, LD, NB);
+            tile_store<T>(bn + tile_off(a, b, NB), NB, acc);
+          }
+        }
+      }
+    }
+  }
+  if (hasL) {
+    T* sl = p.SL + sys * p.sysS + (long long)sepL * NN;
+#pragma unroll
+    for (int q = 0; q < SQ; ++q) {
+      int s = w + NW * q;
+      if (s < NS) {
+        int a = 0;
+        while (s >= TT - a) {
+          s -= TT - a;
+          ++a;
+        }
+        const int b = a + s;
+        tile_store<T>(sl + tile_off(a, b, NB), NB, Sacc[q]);
+      }
+    }
+  }
+}
+
+}  // namespace bss

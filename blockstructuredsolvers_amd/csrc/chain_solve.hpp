@@ -1,0 +1,176 @@
+// chain_solve.hpp -- per-chain forward and backward substitution sweeps of one level.
+//
+// With the factors of chain_factor.hpp the level's solve is, per chain,
+//   forward : y_i = Uinv_i^T (d_i - C_{i-1}^T y_{i-1})          (first block: no C term)
+//             r_R = C_last^T y_last,   r_L = sum_i W_i^T y_i     (reduced right-hand side)
+//   [reduced system over the separators solved by the next level: x_sep]
+//   backward: x_i = Uinv_i (y_i - C_i x_{i+1} - W_i x_L),  x_{last+1} = x_R
+// which replaces the reference's compute_schur_RHS! (src/backbone_rocm.jl:140-154),
+// update_boundary! (src/backbone_rocm.jl:156-170) and cholesky_solve_batched! with
+// nd = 1 (src/backbone_rocm.jl:84-120).  Each wave owns one 16-row tile of the block;
+// the right-hand sides (up to 16 columns) ride in the MFMA's 16-wide N dimension, so
+// one code path serves nrhs = 1 and nrhs = 16.  Factor blocks are read straight from
+// HBM into MFMA operand registers (contiguous 32-byte column segments); only the
+// n x 16 vectors live in LDS.
+#pragma once
+
+#include "common.hpp"
+
+namespace bss {
+
+template <typename T>
+struct SolveLevelArgs {
+  const T* U;   // Uinv blocks (upper 16x16 tiles)
+  const T* C;   // couplings C
+  const T* W;   // panels W
+  T* d;         // level vectors, n x nrhs per block, in place y -> x
+  T* rR;        // forward: C_last^T y_last per separator
+  T* rL;        // forward: sum W^T y per separator
+  const T* xsep;  // backward: next level's solution, n x nrhs per separator
+  const int4* chains;
+  int nrhs;
+  long long sysU, sysC, sysW, sysD, sysR, sysX;  // per-system element strides
+};
+
+template <typename T>
+struct VecPad {
+  static constexpr int value = sizeof(T) == 8 ? 2 : 4;
+};
+
+// zero an LDS vector buffer (NB rows x 16 columns, leading dimension LDV)
+template <typename T, int LDV, int NT>
+__device__ inline void zero_vec(T* v) {
+  for (int i = threadIdx.x; i < 16 * LDV; i += NT) v[i] = T(0);
+}
+
+// d-tile (rows 16w.., all rhs columns) from a global n x nrhs column-major block
+template <typename T, int NB>
+__device__ inline typename Mfma<T>::acc_t load_rhs_tile(const T* g, int w, int nrhs) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+  typename Mfma<T>::acc_t a;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a[v] = (lr < nrhs) ? g[lr * NB + 16 * w + 4 * lk + v] : T(0);
+  return a;
+}
+
+template <typename T, int NB>
+__device__ inline void store_rhs_tile(T* g, int w, int nrhs, const typename Mfma<T>::acc_t& a) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+  if (lr < nrhs) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[lr * NB + 16 * w + 4 * lk + v] = a[v];
+  }
+}
+
+// global n x nrhs block -> LDS vector buffer (columns >= nrhs left untouched)
+template <typename T, int NB, int LDV, int NT>
+__device__ inline void load_vec(const T* g, T* v, int nrhs) {
+  for (int i = threadIdx.x; i < NB * nrhs; i += NT) v[(i / NB) * LDV + (i % NB)] = g[i];
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs<T> p) {
+  constexpr int TT = NB / 16;
+  constexpr int NT = 64 * TT;
+  constexpr int LDV = NB + VecPad<T>::value;
+  constexpr long long NN = (long long)NB * NB;
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T Y[16 * LDV];
+  __shared__ T R[16 * LDV];
+
+  const int4 cd = p.chains[blockIdx.x];
+  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  if (len <= 0) return;
+  const int sys = blockIdx.y;
+  const int nrhs = p.nrhs;
+  const long long ND = (long long)NB * nrhs;
+  const T* U = p.U + sys * p.sysU;
+  const T* C = p.C + sys * p.sysC;
+  const T* W = p.W + sys * p.sysW;
+  T* d = p.d + sys * p.sysD;
+  const int w = wave_id();
+
+  zero_vec<T, LDV, NT>(Y);
+  zero_vec<T, LDV, NT>(R);
+  __syncthreads();
+
+  acc_t accL = Mfma<T>::zero();
+  for (int i = 0; i < len; ++i) {
+    const long long blk = start + i;
+    acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
+    if (i > 0)
+      tile_mma<T, true, false, true>(acc, C + (blk - 1) * NN + 16 * w * NB, NB, Y, LDV, NB);
+    tile_store<T>(R + 16 * w, LDV, acc);
+    __syncthreads();
+    acc = Mfma<T>::zero();
+    for (int t = 0; t <= w; ++t)
+      tile_mma<T, true, false, false>(acc, U + blk * NN + 16 * w * NB + 16 * t, NB, R + 16 * t,
+                                      LDV, 16);
+    store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
+    tile_store<T>(Y + 16 * w, LDV, acc);
+    __syncthreads();
+    if (sepL >= 0)
+      tile_mma<T, true, false, false>(accL, W + blk * NN + 16 * w * NB, NB, Y, LDV, NB);
+  }
+  if (sepR >= 0) {
+    const long long last = start + len - 1;
+    acc_t acc = Mfma<T>::zero();
+    tile_mma<T, true, false, false>(acc, C + last * NN + 16 * w * NB, NB, Y, LDV, NB);
+    store_rhs_tile<T, NB>(p.rR + sys * p.sysR + sepR * ND, w, nrhs, acc);
+  }
+  if (sepL >= 0) store_rhs_tile<T, NB>(p.rL + sys * p.sysR + sepL * ND, w, nrhs, accL);
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs<T> p) {
+  constexpr int TT = NB / 16;
+  constexpr int NT = 64 * TT;
+  constexpr int LDV = NB + VecPad<T>::value;
+  constexpr long long NN = (long long)NB * NB;
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T Xn[16 * LDV];
+  __shared__ T XL[16 * LDV];
+  __shared__ T R[16 * LDV];
+
+  const int4 cd = p.chains[blockIdx.x];
+  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  if (len <= 0) return;
+  const int sys = blockIdx.y;
+  const int nrhs = p.nrhs;
+  const long long ND = (long long)NB * nrhs;
+  const T* U = p.U + sys * p.sysU;
+  const T* C = p.C + sys * p.sysC;
+  const T* W = p.W + sys * p.sysW;
+  T* d = p.d + sys * p.sysD;
+  const T* xs = p.xsep ? p.xsep + sys * p.sysX : nullptr;
+  const int w = wave_id();
+
+  zero_vec<T, LDV, NT>(Xn);
+  zero_vec<T, LDV, NT>(XL);
+  zero_vec<T, LDV, NT>(R);
+  __syncthreads();
+  if (sepR >= 0) load_vec<T, NB, LDV, NT>(xs + sepR * ND, Xn, nrhs);
+  if (sepL >= 0) load_vec<T, NB, LDV, NT>(xs + sepL * ND, XL, nrhs);
+  __syncthreads();
+
+  for (int i = len - 1; i >= 0; --i) {
+    const long long blk = start + i;
+    acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
+    if (i < len - 1 || sepR >= 0)
+      tile_mma<T, false, false, true>(acc, C + blk * NN + 16 * w, NB, Xn, LDV, NB);
+    if (sepL >= 0) tile_mma<T, false, false, true>(acc, W + blk * NN + 16 * w, NB, XL, LDV, NB);
+    tile_store<T>(R + 16 * w, LDV, acc);
+    __syncthreads();
+    acc = Mfma<T>::zero();
+    for (int t = w; t < TT; ++t)
+      tile_mma<T, false, false, false>(acc, U + blk * NN + 16 * t * NB + 16 * w, NB, R + 16 * t,
+                                       LDV, 16);
+    store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
+    tile_store<T>(Xn + 16 * w, LDV, acc);
+    __syncthreads();
+  }
+}
+
+}  // namespace bss

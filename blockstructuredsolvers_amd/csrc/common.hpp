@@ -1,0 +1,108 @@
+// common.hpp -- shared device helpers for the gfx950 block-tridiagonal Cholesky kernels.
+//
+// Every dense product the solver needs is a 16x16 output tile of op(X) * op(Y) where
+// X and Y are column-major blocks (the reference layout, src/gpu_seq.jl:29-43).
+// They map onto the CDNA MFMA 16x16x4 instructions (f64 and f32, wave64):
+//   A operand: lane l supplies P[l%16][k] for k-slot l/16
+//   B operand: lane l supplies Q[k][l%16] for k-slot l/16
+//   D        : lane l holds D[4*(l/16)+v][l%16], v = 0..3
+// The k ordering inside a product is free, so k = k0 + 4*(l/16) + s over the four
+// instructions s = 0..3 of each 16-deep step.  With that ordering a lane's A operand
+// of X^T (column i of X) and its B operand of Y (column j of Y) are 4 consecutive
+// elements of one column, and a D tile is 4 consecutive elements of one output
+// column: loads and stores stay contiguous in the column-major layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bss {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef d4_t acc_t;
+  __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  typedef f4_t acc_t;
+  __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+__device__ inline int wave_id() { return threadIdx.x >> 6; }
+
+template <typename T>
+__device__ inline T rsqrt_t(T x);
+template <>
+__device__ inline double rsqrt_t<double>(double x) {
+  return 1.0 / sqrt(x);
+}
+template <>
+__device__ inline float rsqrt_t<float>(float x) {
+  return 1.0f / sqrtf(x);
+}
+
+// acc += op(X) * op(Y) for one 16x16 output tile, K a multiple of 16.
+//   TA: op(X) = X^T  -> P[i][k] = X[k][i] at xb[i*ldx + k]; else X[i][k] at xb[k*ldx + i]
+//   TB: op(Y) = Y^T  -> Q[k][j] = Y[j][k] at yb[k*ldy + j]; else Y[k][j] at yb[j*ldy + k]
+//   NEG: subtract instead of add (the A operand is negated).
+// xb/yb point at element (0,0) of the operands' 16-row/col panels; they may live in
+// LDS or global memory.
+template <typename T, bool TA, bool TB, bool NEG>
+__device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
+                                int ldx, const T* __restrict__ yb, int ldy, int K) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = k0 + 4 * lk + s;
+      T a = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
+      T b = TB ? yb[k * ldy + lr] : yb[lr * ldy + k];
+      if (NEG) a = -a;
+      acc = Mfma<T>::mma(a, b, acc);
+    }
+  }
+}
+
+// Load / store a D-layout tile from/to a column-major buffer (element (0,0) at p).
+template <typename T>
+__device__ inline typename Mfma<T>::acc_t tile_load(const T* p, int ld) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+  typename Mfma<T>::acc_t a;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a[v] = p[lr * ld + 4 * lk + v];
+  return a;
+}
+
+template <typename T>
+__device__ inline void tile_store(T* p, int ld, const typename Mfma<T>::acc_t& a) {
+  const int l = lane_id();
+  const int lr = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) p[lr * ld + 4 * lk + v] = a[v];
+}
+
+// Order-independent max of non-negative doubles through their IEEE bit patterns
+// (non-negative doubles order like their uint64 encodings), so device-wide norm
+// reductions are deterministic.
+__device__ inline void atomic_max_nonneg(unsigned long long* addr, double v) {
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace bss

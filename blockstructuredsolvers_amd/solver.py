@@ -1,0 +1,199 @@
+"""Host mirror of the reference's batched GPU API on top of libbss.
+
+Reference interface (src/BlockStructuredSolvers.jl:9-11, src/gpu_batched.jl):
+
+    data = initialize_batched(N, n, T, M)      # gpu_batched.jl:165-182
+    copyto!(data.A_tensor, A); copyto!(data.B_tensor, B); copyto!(data.d_tensor, d)
+    factorize!(data)                           # gpu_batched.jl:184-227
+    solve!(data)                               # gpu_batched.jl:229-259
+    x = data.d_list[1:N]
+
+Python has no ``!`` in names: ``factorize`` / ``solve`` (aliases ``factorize_`` /
+``solve_``) play ``factorize!`` / ``solve!``.  The device array type ``M`` of the
+reference is a torch device here (``"cuda"`` = the HIP device); torch only owns the
+memory and supplies the stream, every computation is a libbss call.  Tensors keep the
+reference layout: ``A_tensor`` is (n, n, N_use) with column-major blocks, ``d_tensor``
+is (n, nrhs, N_use); for a batch of ``nsys`` systems a trailing system axis is added.
+
+Like the reference, ``factorize``/``solve`` are asynchronous on the current stream
+and a non-SPD input is not an exception: ``data.info()`` reports it per system.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _dtype_code(T) -> int:
+    if T in (torch.float64, np.float64, float, "float64", "Float64"):
+        return L.BSS_F64
+    if T in (torch.float32, np.float32, "float32", "Float32"):
+        return L.BSS_F32
+    raise TypeError(f"unsupported element type {T!r} (Float32 or Float64)")
+
+
+def _torch_dtype(code: int):
+    return torch.float64 if code == L.BSS_F64 else torch.float32
+
+
+def _stream_handle(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class BlockTriDiagData_batched:
+    """Counterpart of ``BlockTriDiagData_batched`` (src/gpu_batched.jl:1-71): the
+    user-visible tensors plus an opaque libbss plan holding the schedule and the
+    per-level workspace (the reference's LHS/M/F/G/b buffers and NextData chain)."""
+
+    def __init__(self, N: int, n: int, T, device, nrhs: int, nsys: int, mode: str,
+                 schedule: str, chain_len: Optional[int], base_max: Optional[int]):
+        self.n_user = N
+        self.n = n
+        self.nrhs = nrhs
+        self.nsys = nsys
+        self.mode = mode
+        self.schedule = schedule
+        self.device = torch.device(device)
+        code = _dtype_code(T)
+        mode_code = {"plain": L.BSS_MODE_PLAIN, "mixed": L.BSS_MODE_MIXED}[mode]
+        sched_code = {"auto": L.BSS_SCHED_AUTO, "reference": L.BSS_SCHED_REFERENCE}[schedule]
+        plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            L.call("bss_plan_create", N, n, nrhs, nsys, code, mode_code, sched_code,
+                   ctypes.byref(plan))
+        self._plan = plan
+        if chain_len is not None:
+            L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
+        if base_max is not None:
+            L.call("bss_set_option", plan, L.BSS_OPT_BASE_MAX, float(base_max))
+        self.N = self._get(L.BSS_Q_N_USE)
+        self.dtype = _torch_dtype(code)
+        NU = self.N
+        # column-major blocks: storage (S, N, n, n) C-order, exposed as (n, n, N[, S])
+        a = torch.zeros((nsys, NU, n, n), dtype=self.dtype, device=self.device)
+        b = torch.zeros((nsys, max(NU - 1, 0), n, n), dtype=self.dtype, device=self.device)
+        dd = torch.zeros((nsys, NU, nrhs, n), dtype=self.dtype, device=self.device)
+        self._a, self._b, self._d = a, b, dd
+        perm = (3, 2, 1, 0)
+        if nsys == 1:
+            self.A_tensor = a[0].permute(2, 1, 0)
+            self.B_tensor = b[0].permute(2, 1, 0)
+            self.d_tensor = dd[0].permute(2, 1, 0)
+        else:
+            self.A_tensor = a.permute(*perm)
+            self.B_tensor = b.permute(*perm)
+            self.d_tensor = dd.permute(*perm)
+        # identity padding of the blocks the reference pads (gpu_batched.jl:177-179:
+        # blocks N..N_use, 1-based, i.e. from the user's last block on)
+        if NU > N:
+            eye = torch.eye(n, dtype=self.dtype, device=self.device)
+            a[:, N - 1:] = eye
+        with torch.cuda.device(self.device):
+            L.call("bss_plan_bind", plan, a.data_ptr(), b.data_ptr() if b.numel() else None,
+                   dd.data_ptr())
+        self.levels = self._get(L.BSS_Q_LEVELS)
+        self.P = self._get(L.BSS_Q_LEVEL_N + 1) if self.levels > 1 else 0
+        self.nchains = [self._get(L.BSS_Q_LEVEL_CHAINS + k) for k in range(self.levels)]
+
+    # -- reference-style per-block views (gpu_seq.jl:29-43 create_matrix_list) --------
+    @property
+    def A_list(self) -> List[torch.Tensor]:
+        return [self.A_tensor[:, :, i] for i in range(self.N)]
+
+    @property
+    def B_list(self) -> List[torch.Tensor]:
+        return [self.B_tensor[:, :, i] for i in range(self.N - 1)]
+
+    @property
+    def d_list(self) -> List[torch.Tensor]:
+        return [self.d_tensor[:, :, i] for i in range(self.N)]
+
+    def _get(self, q: int) -> int:
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", self._plan, q, ctypes.byref(v))
+        return int(v.value)
+
+    def level_sizes(self) -> List[int]:
+        return [self._get(L.BSS_Q_LEVEL_N + k) for k in range(self.levels)]
+
+    def info(self) -> np.ndarray:
+        """Per-system potrf status (0 ok, else 1-based first failing block)."""
+        out = (ctypes.c_int * self.nsys)()
+        with torch.cuda.device(self.device):
+            L.call("bss_info", self._plan, out)
+        return np.array(out[:], dtype=np.int64)
+
+    def backward_error(self) -> np.ndarray:
+        """Mixed mode: device-measured normwise backward error per (system, column)."""
+        out = (ctypes.c_double * (self.nsys * self.nrhs))()
+        L.call("bss_last_backward_error", self._plan, out)
+        return np.array(out[:]).reshape(self.nsys, self.nrhs)
+
+    @property
+    def iterations(self) -> int:
+        return self._get(L.BSS_Q_LAST_ITERS)
+
+    def synchronize(self) -> None:
+        with torch.cuda.device(self.device):
+            L.call("bss_sync", self._plan)
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            L.lib().bss_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int = 1,
+                       nsys: int = 1, mode: str = "plain", schedule: str = "auto",
+                       chain_len: Optional[int] = None,
+                       base_max: Optional[int] = None) -> BlockTriDiagData_batched:
+    """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
+
+    ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
+    over the reference: ``nrhs`` right-hand-side columns (the reference hard-codes 1,
+    gpu_batched.jl:257), ``nsys`` same-shape systems in one plan, ``mode="mixed"``
+    (FP32 factor + FP64 refinement, T must be Float64) and ``schedule="reference"``
+    (the reference's level schedule and identity padding to N_use) or ``"auto"``."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("initialize_batched needs a HIP device; none is visible")
+    device = torch.device(M) if not isinstance(M, torch.device) else M
+    if device.type != "cuda":
+        raise ValueError(f"M must name a GPU device, got {M!r}")
+    return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
+                                    base_max)
+
+
+def factorize(data: BlockTriDiagData_batched) -> None:
+    """``factorize!(data)`` (src/gpu_batched.jl:184-227), asynchronous."""
+    with torch.cuda.device(data.device):
+        L.call("bss_factorize", data._plan, _stream_handle(data.device))
+
+
+def solve(data: BlockTriDiagData_batched) -> None:
+    """``solve!(data)`` (src/gpu_batched.jl:229-259): solution in place in d_tensor."""
+    with torch.cuda.device(data.device):
+        L.call("bss_solve", data._plan, _stream_handle(data.device))
+
+
+factorize_ = factorize
+solve_ = solve
+
+
+def reference_schedule(N: int):
+    """The reference's find_sequence_upper (src/gpu_batched.jl:155-162) as computed by
+    libbss: (N_use, [(N_i, m_i, P_i) top level first])."""
+    n_use = ctypes.c_int()
+    buf = (ctypes.c_int * (3 * 64))()
+    k = L.call("bss_reference_schedule", N, ctypes.byref(n_use), buf, 64)
+    return n_use.value, [tuple(buf[3 * i:3 * i + 3]) for i in range(k)]

@@ -1,0 +1,150 @@
+/*
+ * bss.h -- C ABI of libbss, the MI355X (gfx950) recursive Schur-complement
+ * block-tridiagonal Cholesky solver.
+ *
+ * The solver replaces the reference's batched GPU path (src/gpu_batched.jl) and its
+ * ROCm backbone (src/backbone_rocm.jl).  Host code (Julia through ccall, Python
+ * through ctypes, C/C++ directly) binds these entry points; all arguments are plain
+ * integers, doubles and device pointers, and no rocBLAS/rocSOLVER is involved.
+ *
+ * System:   M = tridiag(B_{i-1}^T, A_i, B_i), i = 1..N, blocks n x n, SPD.
+ * Layout:   the reference tensors (src/gpu_seq.jl:29-43 create_matrix_list):
+ *           every block column-major with leading dimension n, blocks contiguous,
+ *           A : n*n*N, B : n*n*(N-1), d : n*nrhs*N elements per system; a batch of
+ *           `nsys` same-shape systems stores system s at offset s*(that size).
+ * Precision: dtype BSS_F64 / BSS_F32 for plain solves in that precision; mode
+ *           BSS_MODE_MIXED (dtype must be BSS_F64) factors in FP32 on MFMA and
+ *           refines in FP64 to a normwise backward error target.
+ * Status:   every entry point returns 0 on success, a negative BSS_E* code on an
+ *           argument or HIP error (message via bss_last_error), and never throws.
+ *           Numerical failure (a non-positive pivot) is not an error status: it is
+ *           reported per system by bss_info, like LAPACK's info.
+ * Threads:  calls on one plan must be serialised by the caller; different plans
+ *           may be used from different threads.  Work is enqueued on the stream
+ *           passed in (NULL = the default stream) and the calls return before it
+ *           completes, as the reference's calls do (test/test_rocm.jl:97,101).
+ */
+#ifndef BSS_H_
+#define BSS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bss_plan* bss_plan_t;
+
+enum bss_dtype { BSS_F32 = 0, BSS_F64 = 1 };
+enum bss_mode { BSS_MODE_PLAIN = 0, BSS_MODE_MIXED = 1 };
+/* BSS_SCHED_REFERENCE reproduces the reference's level schedule exactly
+ * (find_sequence_upper, src/gpu_batched.jl:112-162): separators at both ends,
+ * chains of m_i >= 3 blocks, and N padded up to N_use with identity blocks, which
+ * the caller allocates (bss_plan_get(BSS_Q_N_USE)).  BSS_SCHED_AUTO uses this
+ * library's schedule: no padding, chain lengths chosen per level (tuned table or
+ * explicit bss_set_option(BSS_OPT_CHAIN_LEN*)). */
+enum bss_schedule { BSS_SCHED_AUTO = 0, BSS_SCHED_REFERENCE = 1 };
+
+enum bss_status {
+  BSS_OK = 0,
+  BSS_E_ARG = -1,      /* invalid argument */
+  BSS_E_HIP = -2,      /* HIP runtime error */
+  BSS_E_NOMEM = -3,    /* device allocation failed */
+  BSS_E_STATE = -4,    /* call out of order (e.g. solve before factorize) */
+  BSS_E_UNSUPPORTED = -5
+};
+
+/* Plan queries (bss_plan_get). */
+enum bss_query {
+  BSS_Q_N_USE = 0,          /* blocks per system the caller's A/B/d must hold */
+  BSS_Q_LEVELS = 1,         /* recursion levels incl. the single-chain base */
+  BSS_Q_WORKSPACE_BYTES = 2,
+  BSS_Q_LEVEL_N = 16,       /* + level: blocks at that level */
+  BSS_Q_LEVEL_CHAINS = 48,  /* + level: chains (independent partitions) */
+  BSS_Q_LEVEL_MAXLEN = 80,  /* + level: longest chain */
+  BSS_Q_LAST_ITERS = 112,   /* refinement iterations used by the last mixed solve */
+  BSS_Q_REFACTORED = 113    /* separator levels re-factored in FP64 (mixed mode) */
+};
+
+/* Options (bss_set_option, before bss_plan_finalize / first factorize). */
+enum bss_option {
+  BSS_OPT_TOL = 0,          /* mixed: target normwise backward error (default 1e-12) */
+  BSS_OPT_MAX_ITERS = 1,    /* mixed: max refinement iterations (default 10) */
+  BSS_OPT_PIVOT_TOL = 2,    /* mixed: FP32 pivot threshold relative to the diagonal
+                               (default 1e-5); below it the separator levels are
+                               re-factored in FP64 */
+  BSS_OPT_CHAIN_LEN = 3,    /* auto schedule: chain length for every level */
+  BSS_OPT_BASE_MAX = 4      /* auto schedule: largest single-chain base level */
+};
+
+/* Create a plan for `nsys` independent systems of N blocks of size n with `nrhs`
+ * right-hand-side columns.  Replaces initialize_batched (src/gpu_batched.jl:165-182):
+ * the schedule search and the per-level workspace allocation happen here (device
+ * memory is allocated on the current HIP device); the caller owns A, B, d.
+ * Supported n: 16, 32, 48, 64 (FP32 and FP64), 1 <= nrhs <= 16. */
+int bss_plan_create(int N, int n, int nrhs, int nsys, int dtype, int mode, int schedule,
+                    bss_plan_t* out);
+
+/* Set a numeric option (see enum bss_option). */
+int bss_set_option(bss_plan_t plan, int option, double value);
+
+/* Query plan properties (see enum bss_query). */
+int bss_plan_get(bss_plan_t plan, int query, int64_t* value);
+
+/* Bind the caller's device buffers (reference tensors A_tensor, B_tensor, d_tensor
+ * of BlockTriDiagData_batched, src/gpu_batched.jl:14-27).  A and B are read by
+ * bss_factorize; in plain mode they are overwritten with the factors (as the
+ * reference overwrites them); in mixed mode they are left unchanged (the FP64
+ * residual needs them).  d holds the right-hand sides and receives the solution. */
+int bss_plan_bind(bss_plan_t plan, void* A, void* B, void* d);
+
+/* Factorize all systems.  Replaces factorize!(::BlockTriDiagData_batched)
+ * (src/gpu_batched.jl:184-227) and, through it, cholesky_factorize_batched!,
+ * cholesky_solve_batched! (2n RHS) and compute_schur_complement!
+ * (src/backbone_rocm.jl:62-138).  `stream` is a hipStream_t (NULL = default). */
+int bss_factorize(bss_plan_t plan, void* stream);
+
+/* Solve in place in d with the current factors; may be repeated with new d.
+ * Replaces solve!(::BlockTriDiagData_batched) (src/gpu_batched.jl:229-259) and
+ * compute_schur_RHS!, update_boundary!, cholesky_solve_batched! with nd = 1
+ * (src/backbone_rocm.jl:84-170).  In mixed mode this runs FP32 solves inside FP64
+ * iterative refinement and synchronises the stream once per iteration. */
+int bss_solve(bss_plan_t plan, void* stream);
+
+/* Per-system numerical status after the last factorize (synchronises the stream):
+ * info[s] = 0 if system s factored, else the smallest 1-based block index whose
+ * pivot was non-positive.  The reference passes an `info` buffer to potrf and never
+ * reads it (src/backbone_rocm.jl:64-80); this exposes it. */
+int bss_info(bss_plan_t plan, int* info_host);
+
+/* Per-system, per-column backward error of the last mixed-mode solve
+ * (nsys*nrhs doubles, system-major), as measured on the device in FP64. */
+int bss_last_backward_error(bss_plan_t plan, double* berr_host);
+
+/* Wait for all work enqueued by this plan (hipStreamSynchronize on its stream). */
+int bss_sync(bss_plan_t plan);
+
+/* Free the plan and its workspace (not the caller's A, B, d). */
+int bss_plan_destroy(bss_plan_t plan);
+
+/* Copy the calling thread's last error message into buf (always NUL-terminated);
+ * returns the full message length. */
+int bss_last_error(char* buf, size_t len);
+
+/* ---- Schedule introspection (host only, no device work; used by tests) ---- */
+
+/* The reference's find_sequence_upper (src/gpu_batched.jl:155-162): writes N_use and
+ * up to `cap` (N_i, m_i, P_i) triples, top level first, into levels[3*k..3*k+2];
+ * returns the number of levels or a negative status. */
+int bss_reference_schedule(int N, int* n_use, int* levels, int cap);
+
+/* Chain layout of a plan level: for chain c, chains[4c..4c+3] = {start block, length,
+ * has-left-separator, has-right-separator}; returns the number of chains. */
+int bss_plan_level_chains(bss_plan_t plan, int level, int* chains, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSS_H_ */

@@ -1,0 +1,130 @@
+"""GPU parity of the libbss solver against the FP64 CPU oracle (restatement of the
+reference's cpu.jl path) on identical seeded inputs.
+
+Bars (the north star's): plain FP64 -- relative solution difference <= 1e-12 and
+normwise backward error <= 1e-13; plain FP32 -- the reference tests' bound
+||x - x_ref|| <= 10 sqrt(eps32) ||x_ref|| (test/test_rocm.jl:118); mixed -- backward
+error <= 1e-12.
+"""
+import numpy as np
+import pytest
+import torch
+
+from blockstructuredsolvers_amd import factorize, initialize_batched, solve, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+F64_REL = 1e-12
+F64_BERR = 1e-13
+
+
+def run_gpu(sysm, T=torch.float64, **kw):
+    N, n, nrhs = sysm.N, sysm.n, sysm.nrhs
+    data = initialize_batched(N, n, T, "cuda", nrhs=nrhs, **kw)
+    data.A_tensor[:, :, :N].copy_(torch.from_numpy(sysm.A))
+    data.B_tensor[:, :, :N - 1].copy_(torch.from_numpy(sysm.B))
+    data.d_tensor[:, :, :N].copy_(torch.from_numpy(sysm.d))
+    factorize(data)
+    solve(data)
+    torch.cuda.synchronize()
+    x = data.d_tensor[:, :, :N].double().cpu().numpy()
+    return x, data
+
+
+def check_f64(sysm, x):
+    xo, info = oracle.solve_system(sysm.A, sysm.B, sysm.d)
+    assert info == 0
+    rel = np.abs(x - xo).max() / np.abs(xo).max()
+    berr = oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max()
+    assert rel <= F64_REL, f"rel diff {rel:.3e}"
+    assert berr <= F64_BERR, f"backward error {berr:.3e}"
+    return rel, berr
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64])
+@pytest.mark.parametrize("schedule", ["auto", "reference"])
+def test_f64_small(n, schedule):
+    sysm = workloads.spd_system(50, n, nrhs=1, seed=7 + n)
+    x, data = run_gpu(sysm, schedule=schedule)
+    print(f"n={n} schedule={schedule} levels={data.levels}", check_f64(sysm, x))
+    assert data.info()[0] == 0
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 9, 17, 100, 257])
+def test_f64_sizes(N):
+    sysm = workloads.spd_system(N, 32, nrhs=2, seed=N)
+    for schedule in ("auto", "reference"):
+        x, _ = run_gpu(sysm, schedule=schedule)
+        check_f64(sysm, x)
+
+
+def test_f64_config1_reference_schedule():
+    """Config 1 (N=100, n=32, FP64, seed 42): one level (100, 8, 12) + base of 12."""
+    sysm = workloads.spd_system(100, 32, nrhs=1, seed=42)
+    x, data = run_gpu(sysm, schedule="reference")
+    assert data.level_sizes() == [100, 12]
+    check_f64(sysm, x)
+
+
+def test_f64_multilevel_padding():
+    """N=2000 reference schedule pads to N_use and recurses several levels."""
+    sysm = workloads.spd_system(2000, 16, nrhs=1, seed=3)
+    x, data = run_gpu(sysm, schedule="reference")
+    assert data.N >= 2000 and data.levels >= 3
+    check_f64(sysm, x)
+    # the user's d blocks beyond N stay zero-solution for the identity padding
+    pad = data.d_tensor[:, :, 2000:].cpu().numpy()
+    assert np.all(pad == 0)
+
+
+@pytest.mark.parametrize("chain_len", [1, 2, 3, 5, 16])
+def test_f64_chain_lengths(chain_len):
+    sysm = workloads.spd_system(300, 32, nrhs=3, seed=11)
+    x, _ = run_gpu(sysm, chain_len=chain_len, base_max=4)
+    check_f64(sysm, x)
+
+
+def test_f64_multi_rhs16():
+    sysm = workloads.spd_system(200, 64, nrhs=16, seed=5)
+    x, _ = run_gpu(sysm)
+    check_f64(sysm, x)
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_f32_matches_reference_tolerance(n):
+    sysm = workloads.spd_system(50, n, nrhs=1, seed=21, manufactured=True)
+    s32 = workloads.BlockSystem(sysm.A.astype(np.float32, order="F"),
+                                sysm.B.astype(np.float32, order="F"),
+                                sysm.d.astype(np.float32, order="F"))
+    x, _ = run_gpu(s32, T=torch.float32)
+    xo, _ = oracle.solve_system(sysm.A, sysm.B, sysm.d)
+    err = oracle.bss_norm(x - xo)
+    assert err <= 10 * np.sqrt(np.finfo(np.float32).eps) * oracle.bss_norm(xo), err
+
+
+def test_repeated_solves_new_rhs():
+    """solve! may be repeated after one factorize! (factors are not modified)."""
+    sysm = workloads.spd_system(120, 32, nrhs=1, seed=9)
+    x, data = run_gpu(sysm)
+    check_f64(sysm, x)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        d2 = np.asfortranarray(rng.uniform(size=sysm.d.shape))
+        data.d_tensor[:, :, :sysm.N].copy_(torch.from_numpy(d2))
+        solve(data)
+        torch.cuda.synchronize()
+        x2 = data.d_tensor.cpu().numpy()
+        check_f64(workloads.BlockSystem(sysm.A, sysm.B, d2), x2)
+
+
+def test_non_spd_reports_info():
+    sysm = workloads.spd_system(40, 16, nrhs=1, seed=2)
+    A = sysm.A.copy(order="F")
+    A[:, :, 17] = -np.eye(16)  # block 18 (1-based) is negative definite
+    bad = workloads.BlockSystem(A, sysm.B, sysm.d)
+    _, data = run_gpu(bad)
+    info = data.info()[0]
+    assert info > 0
+    ref_info = oracle.factorize(A.copy(order="F"), sysm.B.copy(order="F"))
+    assert ref_info > 0
