@@ -79,13 +79,16 @@ __device__ inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
     const T s = rsqrt_t<T>(p);
     const T ujc = __shfl(a[sv], lr + 16 * sg) * s;
     const T ejc = __shfl(e[sv], lr + 16 * sg) * s;
+    // all of row j is read before any lane overwrites its copy of row j
+    T ujr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ujr[v] = __shfl(a[sv], 4 * lk + v + 16 * sg) * s;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = 4 * lk + v;
-      const T ujr = __shfl(a[sv], r + 16 * sg) * s;
       if (r > j) {
-        a[v] -= ujr * ujc;
-        e[v] -= ujr * ejc;
+        a[v] -= ujr[v] * ujc;
+        e[v] -= ujr[v] * ejc;
       } else if (r == j) {
         a[v] = ujc;
         e[v] = ejc;

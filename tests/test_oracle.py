@@ -1,0 +1,174 @@
+"""Pins the CPU oracle (oracle/bss_oracle.c, the restatement of src/backbone_cpu.jl)
+against independent references: scipy's LAPACK/BLAS for the dense kernels, a dense
+numpy Cholesky solve of the assembled matrix, closed forms, and the reference's own
+test pattern (manufactured solutions with its tolerances).  Each check is chosen so a
+plausible slip -- a transposed coupling, a wrong triangle, a dropped update -- fails
+it; test_planted_mistakes_are_caught plants such slips in a copy of the oracle's
+inputs/outputs and shows the checks reject them."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.linalg import blas, lapack
+
+from blockstructuredsolvers_amd import workloads
+from oracle import oracle
+
+
+def dense(A, B):
+    n, _, N = A.shape
+    M = np.zeros((N * n, N * n))
+    for i in range(N):
+        M[i * n:(i + 1) * n, i * n:(i + 1) * n] = A[:, :, i]
+        if i + 1 < N:
+            M[i * n:(i + 1) * n, (i + 1) * n:(i + 2) * n] = B[:, :, i]
+            M[(i + 1) * n:(i + 2) * n, i * n:(i + 1) * n] = B[:, :, i].T
+    return M
+
+
+def flat(x):
+    n, nrhs, N = x.shape
+    return np.concatenate([x[:, :, i] for i in range(N)], axis=0)
+
+
+@pytest.mark.parametrize("N,n,nrhs", [(1, 5, 1), (2, 3, 2), (9, 4, 1), (50, 8, 3), (20, 32, 2)])
+def test_oracle_vs_dense_cholesky(N, n, nrhs):
+    # A_i = G G^T + alpha n I is SPD with B_i ~ N(0,1) only once n*alpha dominates
+    # ||B||; the survey checked alpha = 1 for n >= 16, tiny blocks need a larger shift
+    s = workloads.spd_system(N, n, nrhs=nrhs, seed=N * 100 + n, alpha=1.0 if n >= 16 else 4.0)
+    x, info = oracle.solve_system(s.A, s.B, s.d)
+    assert info == 0
+    M = dense(s.A, s.B)
+    c = sla.cho_factor(M)
+    xd = sla.cho_solve(c, flat(s.d))
+    assert np.abs(flat(x) - xd).max() <= 1e-12 * np.abs(xd).max()
+
+
+def test_oracle_blocks_vs_lapack():
+    """The factors themselves: U_i from potrf('U') and C_i = U_i^{-T} B_i, checked
+    against scipy's dpotrf/dtrsm applied in the reference's order
+    (backbone_cpu.jl:5-15)."""
+    s = workloads.spd_system(6, 12, seed=3, alpha=4.0)
+    A, B = s.A.copy(order="F"), s.B.copy(order="F")
+    assert oracle.factorize(A, B) == 0
+    Ar, Br = s.A.copy(order="F"), s.B.copy(order="F")
+    U, info = lapack.dpotrf(Ar[:, :, 0], lower=0)
+    assert info == 0
+    Us = [U]
+    Cs = []
+    for i in range(1, 6):
+        C = blas.dtrsm(1.0, Us[-1], Br[:, :, i - 1], side=0, lower=0, trans_a=1)
+        Cs.append(C)
+        Ai = blas.dgemm(-1.0, C, C, 1.0, Ar[:, :, i], trans_a=1)
+        Ui, info = lapack.dpotrf(Ai, lower=0)
+        assert info == 0
+        Us.append(Ui)
+    for i in range(6):
+        assert np.allclose(np.triu(A[:, :, i]), np.triu(Us[i]), rtol=1e-13, atol=1e-13)
+    # potrf('U') leaves the strict lower triangle untouched (gemm updated it for i > 0)
+    assert np.array_equal(np.tril(A[:, :, 0], -1), np.tril(s.A[:, :, 0], -1))
+    low1 = s.A[:, :, 1] - Cs[0].T @ Cs[0]
+    assert np.allclose(np.tril(A[:, :, 1], -1), np.tril(low1, -1), rtol=1e-13, atol=1e-13)
+    for i in range(5):
+        assert np.allclose(B[:, :, i], Cs[i], rtol=1e-12, atol=1e-13)
+
+
+def test_closed_form_scalar_laplacian():
+    """n = 1, A_i = 2, B_i = -1: M = tridiag(-1, 2, -1) and M x = e_1 has the closed
+    form x_i = (N + 1 - i) / (N + 1) (1-based i)."""
+    N = 37
+    A = np.full((1, 1, N), 2.0, order="F")
+    B = np.full((1, 1, N - 1), -1.0, order="F")
+    d = np.zeros((1, 1, N), order="F")
+    d[0, 0, 0] = 1.0
+    x, info = oracle.solve_system(A, B, d)
+    i = np.arange(1, N + 1)
+    assert info == 0
+    assert np.allclose(x[0, 0, :], (N + 1 - i) / (N + 1), rtol=1e-14, atol=1e-15)
+
+
+def test_block_diagonal_reduces_to_independent_solves():
+    s = workloads.spd_system(5, 6, nrhs=2, seed=4, alpha=4.0)
+    B0 = np.zeros_like(s.B, order="F")
+    x, _ = oracle.solve_system(s.A, B0, s.d)
+    for i in range(5):
+        assert np.allclose(x[:, :, i], np.linalg.solve(s.A[:, :, i], s.d[:, :, i]), rtol=1e-13)
+
+
+def test_reference_test_pattern_f64_f32():
+    """test/test_cpu.jl:82-168: manufactured x, d = M x; forward error and residual
+    below 10 sqrt(eps(T)) relative (N=50, n=32 as the reference runs it)."""
+    for dt in (np.float64, np.float32):
+        s = workloads.spd_system(50, 32, seed=42, manufactured=True, dtype=dt)
+        x, info = oracle.solve_system(s.A, s.B, s.d)
+        eps = np.sqrt(np.finfo(dt).eps)
+        assert info == 0
+        assert oracle.bss_norm(x - s.x) <= 10 * eps * oracle.bss_norm(s.x)
+        r = oracle.matvec(s.A, s.B, x) - s.d
+        assert oracle.bss_norm(r) <= 10 * eps * oracle.bss_norm(s.d)
+
+
+def test_backward_error_definition():
+    s = workloads.spd_system(8, 4, nrhs=2, seed=1, alpha=4.0)
+    x, _ = oracle.solve_system(s.A, s.B, s.d)
+    M = dense(s.A, s.B)
+    X, D = flat(x), flat(s.d)
+    R = D - M @ X
+    want = np.abs(R).max(0) / (np.abs(M).sum(1).max() * np.abs(X).max(0) + np.abs(D).max(0))
+    assert np.allclose(oracle.backward_error(s.A, s.B, x, s.d), want, rtol=1e-10)
+    assert want.max() < 1e-15
+
+
+def test_info_first_failing_block():
+    s = workloads.spd_system(10, 4, seed=5, alpha=4.0)
+    A = s.A.copy(order="F")
+    A[:, :, 6] = -np.eye(4)
+    assert oracle.factorize(A, s.B.copy(order="F")) == 7
+
+
+def test_reference_batched_restatement_agrees():
+    """The numpy restatement of gpu_batched.jl (padding, recursion, F = T^{-1} G) gives
+    the sequential oracle's answer, for the reference's own test size and a padded
+    multi-level size."""
+    for N, n in [(50, 32), (100, 8), (300, 4), (5, 6)]:
+        s = workloads.spd_system(N, n, nrhs=2, seed=N, alpha=1.0 if n >= 16 else 4.0)
+        x, _ = oracle.solve_system(s.A, s.B, s.d)
+        xr = oracle.ref_batched_solve_system(s.A, s.B, s.d)
+        assert np.abs(xr - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_reference_schedule_hand_table():
+    """find_sequence_upper (gpu_batched.jl:155-162) at the sizes the survey checked by
+    hand; N_list top level = N_use; N_i = (P_i - 1) m_i + P_i; P_{i+1} = N_i."""
+    table = {
+        50: (50, [(50, 6, 8)]),
+        100: (100, [(100, 8, 12)]),
+        8192: (8193, [(8193, 3, 2049), (2049, 3, 513), (513, 3, 129), (129, 3, 33),
+                      (33, 3, 9), (9, 3, 3)]),
+        4096: (4096, [(4096, 4, 820), (820, 6, 118), (118, 8, 14)]),
+        1000: (1000, [(1000, 8, 112), (112, 36, 4)]),
+    }
+    for N, (n_use, levels) in table.items():
+        u, P, Nl, ml = oracle.ref_find_sequence_upper(N)
+        got = list(zip(Nl, ml, P))[::-1]
+        assert u == n_use and got == levels
+        for Ni, mi, Pi in got:
+            assert Ni == (Pi - 1) * mi + Pi and mi >= 3 and Pi >= 3
+
+
+def test_planted_mistakes_are_caught():
+    """Mutations of the solution the checks above must reject: B used untransposed
+    below the diagonal, and one dropped Schur update."""
+    s = workloads.spd_system(12, 5, nrhs=1, seed=8, alpha=4.0)
+    M = dense(s.A, s.B)
+    # wrong coupling orientation: solve with B^T above the diagonal instead of B
+    Bt = np.asfortranarray(np.transpose(s.B, (1, 0, 2)))
+    x_bad, _ = oracle.solve_system(s.A, Bt, s.d)
+    xd = np.linalg.solve(M, flat(s.d))
+    assert np.abs(flat(x_bad) - xd).max() > 1e-6
+    # dropped Schur update: adding C_3^T C_3 to A_4 cancels the oracle's A_4 -= C_3^T C_3
+    Af, Bf = s.A.copy(order="F"), s.B.copy(order="F")
+    oracle.factorize(Af, Bf)
+    A2 = s.A.copy(order="F")
+    A2[:, :, 4] += Bf[:, :, 3].T @ Bf[:, :, 3]
+    x_bad2, _ = oracle.solve_system(A2, s.B, s.d)
+    assert oracle.backward_error(s.A, s.B, x_bad2, s.d).max() > 1e-6

@@ -1,0 +1,119 @@
+"""Host-side logic of libbss on the CPU: the library loads and exports every symbol
+include/bss.h declares, argument errors come back as statuses, the reference schedule
+it computes matches the restatement of find_sequence_upper, and the auto schedule
+tiles every level into chains and separators."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import blockstructuredsolvers_amd as bss
+from blockstructuredsolvers_amd import _lib as L
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "bss.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bss_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    syms = header_symbols()
+    assert len(syms) >= 13
+    lib = L.lib()
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(L.SIGNATURES)
+
+
+def test_argument_errors():
+    p = ctypes.c_void_p()
+    rc = L.lib().bss_plan_create(10, 24, 1, 1, 1, 0, 0, ctypes.byref(p))
+    assert rc == -5 and "n=24" in L.last_error()
+    rc = L.lib().bss_plan_create(10, 32, 1, 1, 0, 1, 0, ctypes.byref(p))
+    assert rc == -1 and "mixed" in L.last_error()
+    rc = L.lib().bss_plan_create(0, 32, 1, 1, 1, 0, 0, ctypes.byref(p))
+    assert rc == -1
+    rc = L.lib().bss_plan_create(10, 32, 17, 1, 1, 0, 0, ctypes.byref(p))
+    assert rc == -5
+    assert L.lib().bss_factorize(None, None) == -1
+    L.call("bss_plan_create", 10, 32, 1, 1, 1, 0, 0, ctypes.byref(p))
+    try:
+        assert L.lib().bss_factorize(p, None) == -4  # not bound
+        assert L.lib().bss_solve(p, None) == -4      # not factored
+        assert L.lib().bss_set_option(p, 99, 1.0) == -1
+    finally:
+        L.lib().bss_plan_destroy(p)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 9, 16, 50, 100, 257, 1000, 4096, 8192, 10007])
+def test_reference_schedule_matches_restatement(N):
+    n_use, levels = bss.reference_schedule(N)
+    u, P, Nl, ml = oracle.ref_find_sequence_upper(N)
+    assert n_use == u
+    assert levels == list(zip(Nl, ml, P))[::-1]
+
+
+def plan_chains(N, schedule=0, **opt):
+    p = ctypes.c_void_p()
+    L.call("bss_plan_create", N, 64, 1, 1, 1, 0, schedule, ctypes.byref(p))
+    try:
+        for k, v in opt.items():
+            L.call("bss_set_option", p, {"chain_len": L.BSS_OPT_CHAIN_LEN,
+                                         "base_max": L.BSS_OPT_BASE_MAX}[k], float(v))
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_LEVELS, ctypes.byref(v))
+        out = []
+        for k in range(v.value):
+            L.call("bss_plan_get", p, L.BSS_Q_LEVEL_N + k, ctypes.byref(v))
+            NL = v.value
+            buf = (ctypes.c_int * (4 * (NL + 2)))()
+            nch = L.call("bss_plan_level_chains", p, k, buf, NL + 2)
+            out.append((NL, [tuple(buf[4 * c:4 * c + 4]) for c in range(nch)]))
+        L.call("bss_plan_get", p, L.BSS_Q_N_USE, ctypes.byref(v))
+        return v.value, out
+    finally:
+        L.lib().bss_plan_destroy(p)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 7, 8, 9, 10, 31, 100, 1000, 8192])
+@pytest.mark.parametrize("chain_len,base_max", [(3, 8), (1, 2), (2, 4), (7, 16), (64, 8)])
+def test_auto_schedule_tiles_levels(N, chain_len, base_max):
+    n_use, levels = plan_chains(N, chain_len=chain_len, base_max=base_max)
+    assert n_use == N
+    assert levels[0][0] == N
+    for k, (NL, chains) in enumerate(levels):
+        covered = np.zeros(NL, dtype=int)
+        seps = []
+        for (start, ln, hl, hr) in chains:
+            assert ln >= 1
+            covered[start:start + ln] += 1
+            if hr:
+                seps.append(start + ln)
+            if hl:
+                assert start - 1 >= 0
+        for s_ in seps:
+            covered[s_] += 1
+        assert np.all(covered == 1), (k, NL)
+        if k + 1 < len(levels):
+            assert levels[k + 1][0] == len(seps)
+            lens = [c[1] for c in chains]
+            assert max(lens) - min(lens) <= 1  # balanced chains
+            assert max(lens) <= 2 * max(chain_len, 1) + 1
+    assert len(levels[-1][1]) == 1  # base: one chain, no separators
+    assert levels[-1][0] <= max(base_max, 2) or len(levels) == 1
+
+
+def test_reference_schedule_levels_layout():
+    """Reference layout: separators at blocks k(m+1), chains of m between them."""
+    n_use, levels = plan_chains(100, schedule=1)
+    assert n_use == 100 and [lv[0] for lv in levels] == [100, 12]
+    NL, chains = levels[0]
+    assert len(chains) == 11
+    assert all(c == (k * 9 + 1, 8, 1, 1) for k, c in enumerate(chains))
+    assert levels[1][1] == [(0, 12, 0, 0)]
