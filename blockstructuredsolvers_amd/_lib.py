@@ -30,6 +30,10 @@ BSS_OPT_MAX_ITERS = 1
 BSS_OPT_PIVOT_TOL = 2
 BSS_OPT_CHAIN_LEN = 3
 BSS_OPT_BASE_MAX = 4
+BSS_OPT_PRESERVE_INPUT = 5
+
+KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,
+                "backward": 4, "scatter": 5, "residual": 6, "mnorm": 7}
 
 # every symbol include/bss.h declares, with its ctypes signature
 _VP, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
@@ -47,6 +51,8 @@ SIGNATURES = {
     "bss_last_error": ([ctypes.c_char_p, ctypes.c_size_t], _I),
     "bss_reference_schedule": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I], _I),
     "bss_plan_level_chains": ([_VP, _I, ctypes.POINTER(_I), _I], _I),
+    "bss_profile_enable": ([_VP, _I], _I),
+    "bss_profile_read": ([_VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
 }
 
 _lib = None

@@ -77,10 +77,23 @@ struct bss_plan {
   int max_iters = 10;
   std::vector<int> chain_len;
   int base_max = 8;
+  bool preserve = false;  // plain mode: factors to workspace, A/B untouched
   // mixed-precision state
   bss::MixedState mx;
   int last_iters = 0, refactored = 0;
   hipStream_t last_stream = nullptr;
+  // profiler: event pairs around kernel launches while enabled
+  struct ProfRec {
+    int kind, level;
+    hipEvent_t a, b;
+  };
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t pool_next = 0;
+  ~bss_plan() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -88,6 +101,36 @@ namespace {
 size_t fsize(const bss_plan* p) {  // factor element size
   return (p->dtype == BSS_F64 && p->mode == BSS_MODE_PLAIN) ? 8 : 4;
 }
+
+// factors of level 0 live in the caller's A/B (reference semantics) unless the plan
+// is mixed or asked to preserve its input
+bool inplace0(const bss_plan* p) { return p->mode == BSS_MODE_PLAIN && !p->preserve; }
+
+hipEvent_t prof_event(bss_plan* p) {
+  if (p->pool_next == p->pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p->pool.push_back(e);
+  }
+  return p->pool[p->pool_next++];
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct ProfScope {
+  bss_plan* p;
+  hipStream_t s;
+  long idx = -1;
+  ProfScope(bss_plan* p_, int kind, int level, hipStream_t s_) : p(p_), s(s_) {
+    if (!p->prof) return;
+    hipEvent_t a = prof_event(p), b = prof_event(p);
+    if (!a || !b || hipEventRecord(a, s) != hipSuccess) return;
+    p->recs.push_back({kind, level, a, b});
+    idx = (long)p->recs.size() - 1;
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(p->recs[idx].b, s);
+  }
+};
 
 struct Bump {
   size_t off = 0;
@@ -145,6 +188,9 @@ int build(bss_plan* p, bool alloc = true) {
       o.U = bump.take(S * NL * nn * fs);
       o.C = bump.take(S * std::max<size_t>(NL - 1, 1) * nn * fs);
       o.d = bump.take(S * NL * nd * fs);  // FP32 correction vector
+    } else if (p->preserve) {
+      o.U = bump.take(S * NL * nn * fs);
+      o.C = bump.take(S * std::max<size_t>(NL - 1, 1) * nn * fs);
     }
     bool anyL = false;
     for (auto& c : L.shape.chains) anyL = anyL || c.sepL >= 0;
@@ -190,7 +236,7 @@ int build(bss_plan* p, bool alloc = true) {
       L.d = at(o.d);
       L.U = L.A;
       L.C = L.B;
-    } else if (mixed) {
+    } else if (mixed || p->preserve) {
       L.U = at(o.U);
       L.C = at(o.C);
       L.d = at(o.d);
@@ -229,8 +275,8 @@ int factor_levels(bss_plan* p, hipStream_t s) {
     bss::FactorLevelArgs<T> a{};
     a.A_in = (l == 0) ? p->A : L.A;
     a.B_in = (l == 0) ? p->B : L.B;
-    a.U_out = (T*)((l == 0 && p->mode == BSS_MODE_PLAIN) ? p->A : L.U);
-    a.C_out = (T*)((l == 0 && p->mode == BSS_MODE_PLAIN) ? p->B : L.C);
+    a.U_out = (T*)((l == 0 && inplace0(p)) ? p->A : L.U);
+    a.C_out = (T*)((l == 0 && inplace0(p)) ? p->B : L.C);
     a.W_out = (T*)L.W;
     a.SR = (T*)L.SR;
     a.SL = (T*)L.SL;
@@ -246,12 +292,16 @@ int factor_levels(bss_plan* p, hipStream_t s) {
     a.sysS = L.sS;
     a.sysBn = Nx ? Nx->sB : 0;
     hipError_t e;
-    if (l == 0)
-      e = bss::launch_factor<Tin, T>(nb, a, L.nch, p->nsys, s);
-    else
-      e = bss::launch_factor<T, T>(nb, a, L.nch, p->nsys, s);
+    {
+      ProfScope ps(p, BSS_K_FACTOR, (int)l, s);
+      if (l == 0)
+        e = bss::launch_factor<Tin, T>(nb, a, L.nch, p->nsys, s);
+      else
+        e = bss::launch_factor<T, T>(nb, a, L.nch, p->nsys, s);
+    }
     if (e != hipSuccess) return fail(BSS_E_HIP, "factor level %zu launch: %s", l, hipGetErrorString(e));
     if (Nx) {
+      ProfScope ps(p, BSS_K_ASSEMBLE_DIAG, (int)l, s);
       if (l == 0)
         e = bss::launch_assemble_diag<Tin, T>((const Tin*)p->A, (const T*)L.SR, (const T*)L.SL,
                                               L.dseppos, (T*)Nx->A, nb, L.P, p->nsys, L.sA,
@@ -272,9 +322,9 @@ int solve_levels(bss_plan* p, T* d0, hipStream_t s) {
   const int nb = p->n;
   const int nd = p->n * p->nrhs;
   const size_t L0 = p->lv.size();
-  const bool plain = p->mode == BSS_MODE_PLAIN;
-  auto U0 = [&]() -> const T* { return (const T*)(plain ? p->A : p->lv[0].U); };
-  auto C0 = [&]() -> const T* { return (const T*)(plain ? p->B : p->lv[0].C); };
+  const bool inpl = inplace0(p);
+  auto U0 = [&]() -> const T* { return (const T*)(inpl ? p->A : p->lv[0].U); };
+  auto C0 = [&]() -> const T* { return (const T*)(inpl ? p->B : p->lv[0].C); };
   for (size_t l = 0; l < L0; ++l) {
     Level& L = p->lv[l];
     Level* Nx = (l + 1 < L0) ? &p->lv[l + 1] : nullptr;
@@ -292,9 +342,14 @@ int solve_levels(bss_plan* p, T* d0, hipStream_t s) {
     a.sysW = L.sW;
     a.sysD = L.sD;
     a.sysR = L.sR;
-    hipError_t e = bss::launch_forward<T>(nb, a, L.nch, p->nsys, s);
+    hipError_t e;
+    {
+      ProfScope ps(p, BSS_K_FORWARD, (int)l, s);
+      e = bss::launch_forward<T>(nb, a, L.nch, p->nsys, s);
+    }
     if (e != hipSuccess) return fail(BSS_E_HIP, "forward level %zu: %s", l, hipGetErrorString(e));
     if (Nx) {
+      ProfScope ps(p, BSS_K_ASSEMBLE_RHS, (int)l, s);
       e = bss::launch_assemble_rhs<T>(a.d, a.rR, a.rL, L.dseppos, (T*)Nx->d, nd, L.P, p->nsys,
                                       L.sD, L.sR, Nx->sD, s);
       if (e != hipSuccess) return fail(BSS_E_HIP, "rhs assemble %zu: %s", l, hipGetErrorString(e));
@@ -316,9 +371,14 @@ int solve_levels(bss_plan* p, T* d0, hipStream_t s) {
     a.sysW = L.sW;
     a.sysD = L.sD;
     a.sysX = Nx ? Nx->sD : 0;
-    hipError_t e = bss::launch_backward<T>(nb, a, L.nch, p->nsys, s);
+    hipError_t e;
+    {
+      ProfScope ps(p, BSS_K_BACKWARD, (int)li, s);
+      e = bss::launch_backward<T>(nb, a, L.nch, p->nsys, s);
+    }
     if (e != hipSuccess) return fail(BSS_E_HIP, "backward level %zu: %s", li, hipGetErrorString(e));
     if (Nx) {
+      ProfScope ps(p, BSS_K_SCATTER, (int)li, s);
       e = bss::launch_scatter_sep<T>((const T*)Nx->d, L.dseppos, a.d, nd, L.P, p->nsys, Nx->sD,
                                      L.sD, s);
       if (e != hipSuccess) return fail(BSS_E_HIP, "scatter %zu: %s", li, hipGetErrorString(e));
@@ -404,6 +464,10 @@ int bss_set_option(bss_plan_t p, int option, double value) {
       if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "base_max must be >= 1");
       p->base_max = (int)value;
       return BSS_OK;
+    case BSS_OPT_PRESERVE_INPUT:
+      if (p->built) return fail(BSS_E_STATE, "preserve-input must be set before the first bind");
+      p->preserve = value != 0.0;
+      return BSS_OK;
     default:
       return fail(BSS_E_ARG, "unknown option %d", option);
   }
@@ -477,8 +541,12 @@ int bss_factorize(bss_plan_t p, void* stream) {
   int rc;
   if (p->mode == BSS_MODE_MIXED) {
     rc = factor_levels<double, float>(p, s);
-    if (rc == BSS_OK) rc = bss::mixed_after_factor(p->mx, (const double*)p->A,
-                                                   (const double*)p->B, p->n, p->nsys, s);
+    if (rc == BSS_OK) {
+      ProfScope ps(p, BSS_K_MNORM, 0, s);
+      if (bss::mixed_after_factor(p->mx, (const double*)p->A, (const double*)p->B, p->n,
+                                  p->nsys, s) != 0)
+        rc = fail(BSS_E_HIP, "norm kernel: %s", bss::mixed_last_error());
+    }
   } else if (p->dtype == BSS_F64) {
     rc = factor_levels<double, double>(p, s);
   } else {
@@ -499,6 +567,10 @@ int bss_solve(bss_plan_t p, void* stream) {
     hooks.solve32 = [](void* ctx, float* d0, hipStream_t st) -> int {
       return solve_levels<float>((bss_plan*)ctx, d0, st);
     };
+    hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
+      return new ProfScope((bss_plan*)ctx, kind, 0, st);
+    };
+    hooks.prof_end = [](void* scope) { delete (ProfScope*)scope; };
     int iters = 0;
     int rc = bss::mixed_solve(p->mx, hooks, (const double*)p->A, (const double*)p->B,
                               (double*)p->d, (float*)p->lv[0].d, p->n, p->nrhs, p->nsys, p->tol,
@@ -543,6 +615,35 @@ int bss_plan_destroy(bss_plan_t p) {
     (void)hipFree(p->ws);
   }
   delete p;
+  return BSS_OK;
+}
+
+int bss_profile_enable(bss_plan_t p, int on) {
+  if (int e = check_plan(p)) return e;
+  if (on) {
+    if (p->prof) HIP_TRY(hipStreamSynchronize(p->last_stream));
+    p->recs.clear();
+    p->pool_next = 0;
+  }
+  p->prof = on != 0;
+  return BSS_OK;
+}
+
+int bss_profile_read(bss_plan_t p, int kind, int level, int* count, double* total_ms) {
+  if (int e = check_plan(p)) return e;
+  if (!count || !total_ms) return fail(BSS_E_ARG, "null output");
+  HIP_TRY(hipStreamSynchronize(p->last_stream));
+  int c = 0;
+  double t = 0.0;
+  for (const auto& r : p->recs) {
+    if (r.kind != kind || (level >= 0 && r.level != level)) continue;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    ++c;
+    t += ms;
+  }
+  *count = c;
+  *total_ms = t;
   return BSS_OK;
 }
 

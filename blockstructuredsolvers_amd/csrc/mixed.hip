@@ -251,6 +251,19 @@ int mixed_after_factor(MixedState& m, const double* A, const double* B, int n, i
   return 0;
 }
 
+namespace {
+constexpr int kKindResidual = 6;  // BSS_K_RESIDUAL in bss.h
+
+hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, const double* A,
+                         const double* B, const double* d, const double* xold,
+                         const float* dx, double* xnew, float* rout, int init, hipStream_t s) {
+  void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindResidual, s) : nullptr;
+  hipError_t e = residual(m, A, B, d, xold, dx, xnew, rout, init, s);
+  if (scope) h.prof_end(scope);
+  return e;
+}
+}  // namespace
+
 int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const double* B,
                 double* d, float* v0, int n, int nrhs, int nsys, double tol, int max_iters,
                 int* iters, hipStream_t s) {
@@ -258,7 +271,7 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
   (void)nrhs;
   float* v[2] = {v0, m.corr};
   int cur = 0, xa = 0, it = 0;
-  hipError_t e = residual(m, A, B, d, nullptr, nullptr, m.x[0], v[0], 1, s);
+  hipError_t e = residual_prof(m, h, A, B, d, nullptr, nullptr, m.x[0], v[0], 1, s);
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;
@@ -268,7 +281,7 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
       g_mixed_err = "FP32 solve failed";
       return -1;
     }
-    e = residual(m, A, B, d, m.x[xa], v[cur], m.x[1 - xa], v[1 - cur], 0, s);
+    e = residual_prof(m, h, A, B, d, m.x[xa], v[cur], m.x[1 - xa], v[1 - cur], 0, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * nsys * m.nrhs,
                          hipMemcpyDeviceToHost, s);

@@ -35,6 +35,9 @@ struct MixedState {
 struct MixedSolveHooks {
   void* ctx = nullptr;
   int (*solve32)(void* ctx, float* d0, hipStream_t s) = nullptr;
+  // optional profiler brackets around the residual launches (kind = BSS_K_RESIDUAL)
+  void* (*prof_begin)(void* ctx, int kind, hipStream_t s) = nullptr;
+  void (*prof_end)(void* scope) = nullptr;
 };
 
 // ||M||_inf of every system (after the factorisation; A and B are not modified).

@@ -75,7 +75,23 @@ enum bss_option {
                                (default 1e-5); below it the separator levels are
                                re-factored in FP64 */
   BSS_OPT_CHAIN_LEN = 3,    /* auto schedule: chain length for every level */
-  BSS_OPT_BASE_MAX = 4      /* auto schedule: largest single-chain base level */
+  BSS_OPT_BASE_MAX = 4,     /* auto schedule: largest single-chain base level */
+  BSS_OPT_PRESERVE_INPUT = 5 /* plain mode: 1 = keep A and B unchanged, factors go to
+                               the plan's workspace (default 0: overwrite, as the
+                               reference does) */
+};
+
+/* Kernel kinds for the profiler (bss_profile_read). */
+enum bss_kernel_kind {
+  BSS_K_FACTOR = 0,        /* fused chain factorisation, one launch per level */
+  BSS_K_ASSEMBLE_DIAG = 1, /* reduced diagonal assembly */
+  BSS_K_FORWARD = 2,       /* forward substitution sweep */
+  BSS_K_ASSEMBLE_RHS = 3,  /* reduced right-hand side assembly */
+  BSS_K_BACKWARD = 4,      /* backward substitution sweep */
+  BSS_K_SCATTER = 5,       /* separator solution scatter */
+  BSS_K_RESIDUAL = 6,      /* FP64 residual + update (mixed mode) */
+  BSS_K_MNORM = 7,         /* ||M||_inf (mixed mode) */
+  BSS_K_COUNT = 8
 };
 
 /* Create a plan for `nsys` independent systems of N blocks of size n with `nrhs`
@@ -131,6 +147,17 @@ int bss_plan_destroy(bss_plan_t plan);
 /* Copy the calling thread's last error message into buf (always NUL-terminated);
  * returns the full message length. */
 int bss_last_error(char* buf, size_t len);
+
+/* ---- Profiling (no counterpart in the reference, whose benchmarks time whole
+ * factorize!/solve! calls with AMDGPU.@elapsed, benchmark/benchmark_gpu.jl:143-150) ----
+ * While enabled, every kernel launch of the plan is bracketed by HIP events on the
+ * launch stream.  Enabling clears previous records. */
+int bss_profile_enable(bss_plan_t plan, int on);
+
+/* Sum of kernel durations (ms) and launch count recorded since the last enable, for
+ * one kernel kind (enum bss_kernel_kind) at one level (-1 = all levels).
+ * Synchronises the plan's stream. */
+int bss_profile_read(bss_plan_t plan, int kind, int level, int* count, double* total_ms);
 
 /* ---- Schedule introspection (host only, no device work; used by tests) ---- */
 
