@@ -1,0 +1,315 @@
+"""Headline benchmark: systems/s of the mixed-precision refined solver vs the plain FP64
+GPU path at normwise backward error <= 1e-12, plus p50/p99 solve latency.
+
+Workload: block-tridiagonal SPD systems of N=8192 blocks of n=64 (A_i = G G^T + n I,
+B_i ~ N(0,1), d ~ U(0,1), one seed per system), one right-hand side, the size the
+mixed-precision target is quoted on.  A step factorizes and solves `--nsys` resident
+systems (mixed: FP32 MFMA factorisation + FP64 refinement until every system's
+backward error is <= 1e-12; plain: one FP64 factorize + solve).  Inputs are resident
+in HBM before timing starts; the plans keep A and B unchanged so every step does the
+full work on the same inputs; each step restores the right-hand side (a 4 MB device
+copy per system) before solving.
+
+One JSON line on stdout.  Only one GPU is used: --gpus > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_systems(N, n, nrhs, nsys, seed):
+    from blockstructuredsolvers_amd import workloads
+    return [workloads.spd_system(N, n, nrhs=nrhs, seed=seed + s) for s in range(nsys)]
+
+
+def upload(data, systems, torch):
+    nsys = len(systems)
+    for s, sysm in enumerate(systems):
+        if nsys == 1:
+            data.A_tensor.copy_(torch.from_numpy(sysm.A))
+            data.B_tensor.copy_(torch.from_numpy(sysm.B))
+            data.d_tensor.copy_(torch.from_numpy(sysm.d))
+        else:
+            data.A_tensor[..., s].copy_(torch.from_numpy(sysm.A))
+            data.B_tensor[..., s].copy_(torch.from_numpy(sysm.B))
+            data.d_tensor[..., s].copy_(torch.from_numpy(sysm.d))
+
+
+def kernel_totals(data):
+    """{(kind, level): (count, total_ms)} recorded by the plan's profiler."""
+    from blockstructuredsolvers_amd import _lib as L
+    out = {}
+    for name, kind in L.KERNEL_KINDS.items():
+        for level in range(data.levels):
+            c, t = ctypes.c_int(), ctypes.c_double()
+            L.call("bss_profile_read", data._plan, kind, level, ctypes.byref(c), ctypes.byref(t))
+            if c.value:
+                out[(name, level)] = (c.value, t.value)
+    return out
+
+
+def profile(data, on):
+    from blockstructuredsolvers_amd import _lib as L
+    L.call("bss_profile_enable", data._plan, 1 if on else 0)
+
+
+def run_path(mode, systems, args, torch, bss):
+    """Build a plan, warm up, time `steps` steps; returns a result dict."""
+    N, n, nrhs, nsys = args.N, args.n, args.nrhs, len(systems)
+    from blockstructuredsolvers_amd import _lib as L
+    data = bss.initialize_batched(N, n, torch.float64, "cuda", nrhs=nrhs, nsys=nsys, mode=mode,
+                                  schedule=args.schedule, chain_len=args.chain_len,
+                                  base_max=args.base_max)
+    if mode == "plain":
+        # factors to the workspace so A, B stay resident inputs across steps
+        L.call("bss_set_option", data._plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
+    else:
+        L.call("bss_set_option", data._plan, L.BSS_OPT_TOL, args.tol)
+    upload(data, systems, torch)
+    d0 = data._d.clone()
+    torch.cuda.synchronize()
+
+    def step():
+        data._d.copy_(d0)
+        bss.factorize(data)
+        bss.solve(data)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    profile(data, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = kernel_totals(data)
+    profile(data, False)
+    iters = data.iterations if mode == "mixed" else 0
+    berr_dev = data.backward_error().max() if mode == "mixed" else None
+    x0 = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
+
+    # latency: repeated solves with resident factors (new RHS each time)
+    lat = []
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for k in range(args.lat_solves):
+        data._d.copy_(d0)
+        torch.cuda.synchronize()
+        ev0.record()
+        bss.solve(data)
+        ev1.record()
+        torch.cuda.synchronize()
+        lat.append(ev0.elapsed_time(ev1))
+    res = {
+        "mode": mode,
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "systems_per_s": nsys * args.steps / wall,
+        "iterations": iters,
+        "device_berr": berr_dev,
+        "solve_ms_p50": float(np.percentile(lat, 50)) if lat else None,
+        "solve_ms_p99": float(np.percentile(lat, 99)) if lat else None,
+        "kernels": kt,
+        "levels": data.level_sizes(),
+        "x0": x0,
+    }
+    data.close()
+    return res
+
+
+def level0_blocks(N, levels):
+    """interior (factored) blocks of the top level = N - separators"""
+    return N - (levels[1] if len(levels) > 1 else 0)
+
+
+def roofline_for(res, args, nsys):
+    """Roofline entry for the kernel with the largest total time in the timed region."""
+    kt = res["kernels"]
+    (name, level), (count, total) = max(kt.items(), key=lambda kv: kv[1][1])
+    N, n, nrhs = args.N, args.n, args.nrhs
+    mixed = res["mode"] == "mixed"
+    sf = 4 if mixed else 8           # factor element size
+    NL = res["levels"][level]
+    P = res["levels"][level + 1] if level + 1 < len(res["levels"]) else 0
+    blocks = (NL - P) * nsys          # blocks eliminated by one launch of this level
+    avg_ms = total / count
+    if name == "factor":
+        s_in = 8 if level == 0 else sf
+        # algorithmic: read A, B; write U, C (SURVEY 4 N n^2 s, with input/factor sizes)
+        bytes_per = 2 * n * n * s_in + 2 * n * n * sf
+        flops_per = 7.0 / 3.0 * n ** 3
+        alg_bytes = blocks * bytes_per
+        alg_flops = blocks * flops_per
+        peak_tf = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+        intensity = flops_per / bytes_per
+        ridge = peak_tf * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if intensity >= ridge:
+            ach = alg_flops / (avg_ms * 1e-3) / 1e12
+            return dict(kernel=f"chain_factor L{level}", bound="mfma", achieved=ach, peak=peak_tf,
+                        unit="TFLOP/s", frac=ach / peak_tf, traffic=None,
+                        per_launch_ms=avg_ms, launches=count)
+    elif name in ("forward", "backward"):
+        # read Uinv (upper triangle), C, W and the vectors; write the vectors
+        tri = n * (n + 1) // 2
+        bytes_per = (tri + 2 * n * n) * sf + 2 * n * nrhs * sf
+        alg_bytes = blocks * bytes_per
+    elif name == "residual":
+        bytes_per = 2 * n * n * 8 + n * nrhs * (8 * 3 + 4 * 2)
+        alg_bytes = NL * nsys * bytes_per
+    else:
+        alg_bytes = 0
+    ach = alg_bytes / (avg_ms * 1e-3) / 1e9
+    return dict(kernel=f"{name} L{level}", bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None, per_launch_ms=avg_ms,
+                launches=count)
+
+
+def cpu_baseline(system, seconds, threads):
+    """The FP64 oracle (C restatement of the reference's cpu.jl path) solving copies of
+    one system, one system per thread, until `seconds` of wall time have passed."""
+    from oracle import oracle
+    done = [0] * threads
+    stop = time.perf_counter() + seconds
+    copies = [(system.A.copy(order="F"), system.B.copy(order="F"), system.d.copy(order="F"))
+              for _ in range(threads)]
+
+    def work(k):
+        A0, B0, d0 = copies[k]
+        while time.perf_counter() < stop:
+            A, B, d = A0.copy(order="F"), B0.copy(order="F"), d0.copy(order="F")
+            oracle.factorize(A, B)
+            oracle.solve(A, B, d)
+            done[k] += 1
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    el = time.perf_counter() - t0
+    return sum(done) / el, sum(done), el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=8192)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--nsys", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--schedule", default="auto", choices=["auto", "reference"])
+    ap.add_argument("--chain-len", type=int, default=None)
+    ap.add_argument("--base-max", type=int, default=None)
+    ap.add_argument("--lat-solves", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--modes", default="plain,mixed")
+    ap.add_argument("--seed", type=int, default=42)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          f"(got {args.gpus})"}))
+        sys.exit(2)
+
+    import torch
+    import blockstructuredsolvers_amd as bss
+    from oracle import oracle
+
+    log(f"bench: generating {args.nsys} systems N={args.N} n={args.n} nrhs={args.nrhs}")
+    systems = make_systems(args.N, args.n, args.nrhs, args.nsys, args.seed)
+    results = {}
+    for mode in args.modes.split(","):
+        log(f"bench: {mode} path")
+        results[mode] = run_path(mode, systems, args, torch, bss)
+        r = results[mode]
+        log(f"bench: {mode}: {r['systems_per_s']:.1f} systems/s, {r['ms_per_step']:.3f} ms/step, "
+            f"solve p50 {r['solve_ms_p50']:.3f} ms p99 {r['solve_ms_p99']:.3f} ms, "
+            f"iters {r['iterations']}")
+        for (k, lvl), (c, t) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][1])[:8]:
+            log(f"    {k:14s} L{lvl}: {c:4d} launches, {t / c:.4f} ms avg, {t:.3f} ms total")
+
+    acc = {}
+    if not args.no_oracle:
+        log("bench: FP64 oracle on system 0")
+        s0 = systems[0]
+        xo, info = oracle.solve_system(s0.A, s0.B, s0.d)
+        for mode, r in results.items():
+            x = r["x0"]
+            acc[mode] = {
+                "rel_diff_vs_oracle": float(np.abs(x - xo).max() / np.abs(xo).max()),
+                "backward_error": float(oracle.backward_error(s0.A, s0.B, x, s0.d).max()),
+            }
+        log(f"bench: accuracy {acc}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline, {threads} threads, {args.cpu_seconds}s")
+        v, count, el = cpu_baseline(systems[0], args.cpu_seconds, threads)
+        cpu = {"value": v, "unit": "systems/s", "cores": threads, "kind": "port",
+               "sample": f"{count} FP64 factorize+solve of one N={args.N}, n={args.n}, "
+                         f"nrhs={args.nrhs} system in {el:.1f} s, one system per thread "
+                         "(C restatement of cpu.jl, oracle/bss_oracle.c)"}
+
+    head = results.get("mixed") or results.get("plain")
+    plain = results.get("plain")
+    roof = roofline_for(head, args, args.nsys)
+    out = {
+        "metric": "systems/s, mixed vs plain FP64 at <=1e-12 backward err; p50/p99 solve ms",
+        "value": head["systems_per_s"],
+        "unit": "systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factor + f64 refinement" if head["mode"] == "mixed" else "f64",
+        "data": "synthetic (seeded A_i = G G^T + n I, B_i ~ N(0,1), d ~ U(0,1))",
+        "config": {"workload": f"block-tridiagonal SPD, N={args.N} blocks of n={args.n}, "
+                               f"nrhs={args.nrhs}, {args.nsys} systems per step",
+                   "N": args.N, "n": args.n, "nrhs": args.nrhs, "systems_per_step": args.nsys,
+                   "schedule": args.schedule, "levels": head["levels"], "tol": args.tol},
+        "mixed_systems_per_s": results["mixed"]["systems_per_s"] if "mixed" in results else None,
+        "plain_fp64_systems_per_s": plain["systems_per_s"] if plain else None,
+        "mixed_over_plain": (results["mixed"]["systems_per_s"] / plain["systems_per_s"]
+                             if plain and "mixed" in results else None),
+        "refinement_iterations": results["mixed"]["iterations"] if "mixed" in results else None,
+        "solve_ms_p50": {m: r["solve_ms_p50"] for m, r in results.items()},
+        "solve_ms_p99": {m: r["solve_ms_p99"] for m, r in results.items()},
+        "accuracy_vs_fp64_oracle": acc,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
