@@ -50,7 +50,8 @@ __device__ inline typename Mfma<T>::acc_t load_rhs_tile(const T* g, int w, int n
   const int lr = l & 15, lk = l >> 4;
   typename Mfma<T>::acc_t a;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) a[v] = (lr < nrhs) ? g[lr * NB + 16 * w + 4 * lk + v] : T(0);
+  for (int v = 0; v < 4; ++v)
+    a[v] = (lr < nrhs) ? g[lr * NB + 16 * w + Mfma<T>::row(lk, v)] : T(0);
   return a;
 }
 
@@ -60,7 +61,7 @@ __device__ inline void store_rhs_tile(T* g, int w, int nrhs, const typename Mfma
   const int lr = l & 15, lk = l >> 4;
   if (lr < nrhs) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) g[lr * NB + 16 * w + 4 * lk + v] = a[v];
+    for (int v = 0; v < 4; ++v) g[lr * NB + 16 * w + Mfma<T>::row(lk, v)] = a[v];
   }
 }
 

@@ -5,12 +5,15 @@
 // They map onto the CDNA MFMA 16x16x4 instructions (f64 and f32, wave64):
 //   A operand: lane l supplies P[l%16][k] for k-slot l/16
 //   B operand: lane l supplies Q[k][l%16] for k-slot l/16
-//   D        : lane l holds D[4*(l/16)+v][l%16], v = 0..3
+//   D        : lane l, register v (0..3) holds D[row][l%16] with
+//              row = 4*(l/16) + v for f32  and  row = l/16 + 4*v for f64
+//              (the f64 mapping was measured on gfx950 by tests/test_gpu_tiles.py)
 // The k ordering inside a product is free, so k = k0 + 4*(l/16) + s over the four
 // instructions s = 0..3 of each 16-deep step.  With that ordering a lane's A operand
 // of X^T (column i of X) and its B operand of Y (column j of Y) are 4 consecutive
-// elements of one column, and a D tile is 4 consecutive elements of one output
-// column: loads and stores stay contiguous in the column-major layout.
+// elements of one column; D tiles are stored column by column (32-byte runs of 4
+// rows across the lanes of a column), so loads and stores stay contiguous in the
+// column-major layout.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +30,8 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   typedef d4_t acc_t;
+  // row of the 16x16 result held by register v of a lane with l/16 == lk
+  __device__ static constexpr int row(int lk, int v) { return lk + 4 * v; }
   __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
   __device__ static inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -36,6 +41,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   typedef f4_t acc_t;
+  __device__ static constexpr int row(int lk, int v) { return 4 * lk + v; }
   __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static inline acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -86,7 +92,7 @@ __device__ inline typename Mfma<T>::acc_t tile_load(const T* p, int ld) {
   const int lr = l & 15, lk = l >> 4;
   typename Mfma<T>::acc_t a;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) a[v] = p[lr * ld + 4 * lk + v];
+  for (int v = 0; v < 4; ++v) a[v] = p[lr * ld + Mfma<T>::row(lk, v)];
   return a;
 }
 
@@ -95,7 +101,7 @@ __device__ inline void tile_store(T* p, int ld, const typename Mfma<T>::acc_t& a
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) p[lr * ld + 4 * lk + v] = a[v];
+  for (int v = 0; v < 4; ++v) p[lr * ld + Mfma<T>::row(lk, v)] = a[v];
 }
 
 // Order-independent max of non-negative doubles through their IEEE bit patterns

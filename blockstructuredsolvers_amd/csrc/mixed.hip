@@ -64,9 +64,10 @@ __global__ void __launch_bounds__(4 * NB)
     acc_t acc;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      acc[v] = (lr < nrhs) ? ds[(long long)blk * ND + lr * NB + 16 * w + 4 * lk + v] : 0.0;
+      const int row = 16 * w + Mfma<double>::row(lk, v);
+      acc[v] = (lr < nrhs) ? ds[(long long)blk * ND + lr * NB + row] : 0.0;
       dmax = nmax(dmax, fabs(acc[v]));
-      xmax = nmax(xmax, fabs(X[1][lr * LDV + 16 * w + 4 * lk + v]));
+      xmax = nmax(xmax, fabs(X[1][lr * LDV + row]));
     }
     tile_mma<double, false, false, true>(acc, As + blk * NN + 16 * w, NB, X[1], LDV, NB);
     if (blk > 0)
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(4 * NB)
     if (lr < nrhs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        ro[(long long)blk * ND + lr * NB + 16 * w + 4 * lk + v] = (float)acc[v];
+        ro[(long long)blk * ND + lr * NB + 16 * w + Mfma<double>::row(lk, v)] = (float)acc[v];
         rmax = nmax(rmax, fabs(acc[v]));
       }
     }
