@@ -77,14 +77,11 @@ def run_path(mode, systems, args, torch, bss):
     """Build a plan, warm up, time `steps` steps; returns a result dict."""
     N, n, nrhs, nsys = args.N, args.n, args.nrhs, len(systems)
     from blockstructuredsolvers_amd import _lib as L
+    # plain: factors to the workspace so A, B stay resident inputs across steps
     data = bss.initialize_batched(N, n, torch.float64, "cuda", nrhs=nrhs, nsys=nsys, mode=mode,
                                   schedule=args.schedule, chain_len=args.chain_len,
-                                  base_max=args.base_max)
-    if mode == "plain":
-        # factors to the workspace so A, B stay resident inputs across steps
-        L.call("bss_set_option", data._plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
-    else:
-        L.call("bss_set_option", data._plan, L.BSS_OPT_TOL, args.tol)
+                                  base_max=args.base_max, preserve_input=(mode == "plain"),
+                                  tol=args.tol if mode == "mixed" else None)
     upload(data, systems, torch)
     d0 = data._d.clone()
     torch.cuda.synchronize()

@@ -51,7 +51,9 @@ class BlockTriDiagData_batched:
     per-level workspace (the reference's LHS/M/F/G/b buffers and NextData chain)."""
 
     def __init__(self, N: int, n: int, T, device, nrhs: int, nsys: int, mode: str,
-                 schedule: str, chain_len: Optional[int], base_max: Optional[int]):
+                 schedule: str, chain_len: Optional[int], base_max: Optional[int],
+                 preserve_input: bool = False, tol: Optional[float] = None,
+                 max_iters: Optional[int] = None):
         self.n_user = N
         self.n = n
         self.nrhs = nrhs
@@ -71,6 +73,12 @@ class BlockTriDiagData_batched:
             L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
         if base_max is not None:
             L.call("bss_set_option", plan, L.BSS_OPT_BASE_MAX, float(base_max))
+        if preserve_input:
+            L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
+        if tol is not None:
+            L.call("bss_set_option", plan, L.BSS_OPT_TOL, float(tol))
+        if max_iters is not None:
+            L.call("bss_set_option", plan, L.BSS_OPT_MAX_ITERS, float(max_iters))
         self.N = self._get(L.BSS_Q_N_USE)
         self.dtype = _torch_dtype(code)
         NU = self.N
@@ -157,21 +165,25 @@ class BlockTriDiagData_batched:
 def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int = 1,
                        nsys: int = 1, mode: str = "plain", schedule: str = "auto",
                        chain_len: Optional[int] = None,
-                       base_max: Optional[int] = None) -> BlockTriDiagData_batched:
+                       base_max: Optional[int] = None, preserve_input: bool = False,
+                       tol: Optional[float] = None,
+                       max_iters: Optional[int] = None) -> BlockTriDiagData_batched:
     """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
 
     ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
     over the reference: ``nrhs`` right-hand-side columns (the reference hard-codes 1,
     gpu_batched.jl:257), ``nsys`` same-shape systems in one plan, ``mode="mixed"``
     (FP32 factor + FP64 refinement, T must be Float64) and ``schedule="reference"``
-    (the reference's level schedule and identity padding to N_use) or ``"auto"``."""
+    (the reference's level schedule and identity padding to N_use) or ``"auto"``;
+    ``preserve_input`` keeps A and B unchanged in plain mode (factors go to the plan's
+    workspace); ``tol``/``max_iters`` control mixed-mode refinement."""
     if not torch.cuda.is_available():
         raise RuntimeError("initialize_batched needs a HIP device; none is visible")
     device = torch.device(M) if not isinstance(M, torch.device) else M
     if device.type != "cuda":
         raise ValueError(f"M must name a GPU device, got {M!r}")
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
-                                    base_max)
+                                    base_max, preserve_input, tol, max_iters)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:
