@@ -54,11 +54,80 @@ struct LdsPad {
 
 __device__ __host__ constexpr int tile_off(int r, int c, int ld) { return 16 * c * ld + 16 * r; }
 
+// Cross-lane helpers for the diagonal-tile routine.
+template <typename T>
+__device__ inline T read_lane(T x, int lane);
+template <>
+__device__ inline double read_lane<double>(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+template <>
+__device__ inline float read_lane<float>(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// every lane takes the value of lane J of its 16-lane row (DPP row_newbcast)
+template <int J>
+__device__ inline double row_bcast(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x150 + J, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x150 + J, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+template <int J>
+__device__ inline float row_bcast(float x) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, 0xf, 0xf, false));
+}
+
+// 1/sqrt(p): hardware estimate refined by Newton steps (two for f64, one for f32),
+// avoiding the long correctly-rounded sqrt + divide sequences on the serial path.
+__device__ inline double rsqrt_nr(double p) {
+  double y = __builtin_amdgcn_rsq(p);
+  y = y * (1.5 - 0.5 * p * y * y);
+  y = y * (1.5 - 0.5 * p * y * y);
+  return y;
+}
+__device__ inline float rsqrt_nr(float p) {
+  float y = __builtin_amdgcn_rsqf(p);
+  return y * (1.5f - 0.5f * p * y * y);
+}
+
+// One elimination step J of the 16x16 tile (see diag_chol_inv16).
+template <typename T, int J>
+__device__ inline void diag_step(T (&a)[4], T (&e)[4], bool& ok, int lr, int lk) {
+  constexpr int sg = J >> 2, sv = J & 3;
+  const T p = read_lane(a[sv], J + 16 * sg);  // pivot A[J][J]
+  ok = ok && (p > T(0));
+  const T s = rsqrt_nr(p);
+  // row J for the lane's column: A[J][lr] lives in lane lr + 16*sg
+  const T ujc = __shfl(a[sv], lr + 16 * sg) * s;
+  const T ejc = __shfl(e[sv], lr + 16 * sg) * s;
+  // A[J][r] for the lane's rows r = 4lk+v: by symmetry of the trailing matrix it is
+  // A[r][J], column J of the same row group -> a row broadcast of lane J
+  T ujr[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) ujr[v] = row_bcast<J>(a[v]) * s;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 4 * lk + v;
+    if (r > J) {
+      a[v] -= ujr[v] * ujc;
+      e[v] -= ujr[v] * ejc;
+    } else if (r == J) {
+      a[v] = ujc;
+      e[v] = ejc;
+    }
+  }
+  if constexpr (J < 15) diag_step<T, J + 1>(a, e, ok, lr, lk);
+}
+
 // Wave-level Cholesky + inverse of one 16x16 SPD tile held (upper triangle) in LDS.
-// Lane l owns column l%16, rows 4*(l/16)..+3.  Row operations are applied to
-// [A | I]; at the end the right half is U^{-T}.  Writes Uinv = (U^{-T})^T into the
-// upper triangle of the output tile (strict lower part exact zeros).  Returns false
-// if a pivot was not positive.
+// Lane l owns column l%16, rows 4*(l/16)..+3, of the full symmetric tile.  Row
+// operations are applied to [A | I]; at the end the right half is U^{-T}.  Writes
+// Uinv = (U^{-T})^T into the upper triangle of the output tile (strict lower part
+// exact zeros).  Returns false if a pivot was not positive.
 template <typename T>
 __device__ inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
   const int l = lane_id();
@@ -71,30 +140,7 @@ __device__ inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
     e[v] = (r == lr) ? T(1) : T(0);
   }
   bool ok = true;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int sg = j >> 2, sv = j & 3;
-    const T p = __shfl(a[sv], j + 16 * sg);
-    ok = ok && (p > T(0));
-    const T s = rsqrt_t<T>(p);
-    const T ujc = __shfl(a[sv], lr + 16 * sg) * s;
-    const T ejc = __shfl(e[sv], lr + 16 * sg) * s;
-    // all of row j is read before any lane overwrites its copy of row j
-    T ujr[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) ujr[v] = __shfl(a[sv], 4 * lk + v + 16 * sg) * s;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = 4 * lk + v;
-      if (r > j) {
-        a[v] -= ujr[v] * ujc;
-        e[v] -= ujr[v] * ejc;
-      } else if (r == j) {
-        a[v] = ujc;
-        e[v] = ejc;
-      }
-    }
-  }
+  diag_step<T, 0>(a, e, ok, lr, lk);
 #pragma unroll
   for (int v = 0; v < 4; ++v) ut[(4 * lk + v) * ldu + lr] = e[v];
   return ok;
@@ -122,13 +168,13 @@ __device__ inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
       acc_t acc = Mfma<T>::zero();
       if (task < nrow) {
         const int j = kk + 1 + task;
-        tile_mma<T, true, false, false>(acc, Ubuf + tile_off(kk, kk, LD), LD,
-                                        Abuf + tile_off(kk, j, LD), LD, 16);
+        tile_mma<T, true, false, false, 16>(acc, Ubuf + tile_off(kk, kk, LD), LD,
+                                        Abuf + tile_off(kk, j, LD), LD);
         tile_store<T>(Abuf + tile_off(kk, j, LD), LD, acc);
       } else {
         const int t = task - nrow;
-        tile_mma<T, false, false, false>(acc, Ubuf + tile_off(t, kk, LD), LD,
-                                         Ubuf + tile_off(kk, kk, LD), LD, 16);
+        tile_mma<T, false, false, false, 16>(acc, Ubuf + tile_off(t, kk, LD), LD,
+                                         Ubuf + tile_off(kk, kk, LD), LD);
         tile_store<T>(Ubuf + tile_off(t, kk, LD), LD, acc);
       }
     }
@@ -146,15 +192,15 @@ __device__ inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
           }
           const int i = kk + 1 + ii, j = i + idx;
           acc_t acc = tile_load<T>(Abuf + tile_off(i, j, LD), LD);
-          tile_mma<T, true, false, true>(acc, Abuf + tile_off(kk, i, LD), LD,
-                                         Abuf + tile_off(kk, j, LD), LD, 16);
+          tile_mma<T, true, false, true, 16>(acc, Abuf + tile_off(kk, i, LD), LD,
+                                         Abuf + tile_off(kk, j, LD), LD);
           tile_store<T>(Abuf + tile_off(i, j, LD), LD, acc);
         } else {
           const int e = task - nA;
           const int t = e % (kk + 1), i = kk + 1 + e / (kk + 1);
           acc_t acc = (t == kk) ? Mfma<T>::zero() : tile_load<T>(Ubuf + tile_off(t, i, LD), LD);
-          tile_mma<T, false, false, true>(acc, Ubuf + tile_off(t, kk, LD), LD,
-                                          Abuf + tile_off(kk, i, LD), LD, 16);
+          tile_mma<T, false, false, true, 16>(acc, Ubuf + tile_off(t, kk, LD), LD,
+                                          Abuf + tile_off(kk, i, LD), LD);
           tile_store<T>(Ubuf + tile_off(t, i, LD), LD, acc);
         }
       }
@@ -163,18 +209,36 @@ __device__ inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
   }
 }
 
-// global (Tin) NB x NB block -> LDS (T), optionally transposed
-template <typename Tin, typename T, int NB, int LD, int NT, bool TRANS>
-__device__ inline void load_block(const Tin* __restrict__ g, T* s) {
-  for (int idx = threadIdx.x; idx < NB * NB; idx += NT) {
-    const int r = idx % NB, c = idx / NB;
-    const T v = (T)g[idx];
-    if (TRANS)
-      s[r * LD + c] = v;
-    else
-      s[c * LD + r] = v;
+// One NB x NB block staged through registers: every thread issues all of its 16-byte
+// global loads at once (load), and writes them to LDS later (store), so the HBM
+// latency of the next block hides behind the current block's factorisation.
+template <typename Tin, int NB, int NT>
+struct BlockStage {
+  typedef Tin vec_t __attribute__((ext_vector_type(16 / sizeof(Tin))));
+  static constexpr int VEC = 16 / sizeof(Tin);
+  static constexpr int PER = NB * NB / (NT * VEC);
+  static_assert(PER * NT * VEC == NB * NB, "block does not split evenly over the threads");
+  vec_t r[PER];
+  __device__ inline void load(const Tin* __restrict__ g) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) r[q] = *(const vec_t*)(g + (q * NT + (int)threadIdx.x) * VEC);
   }
-}
+  template <typename T, int LD, bool TRANS>
+  __device__ inline void store(T* s) const {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const int idx = (q * NT + (int)threadIdx.x) * VEC + e;
+        const int row = idx % NB, col = idx / NB;
+        if (TRANS)
+          s[row * LD + col] = (T)r[q][e];
+        else
+          s[col * LD + row] = (T)r[q][e];
+      }
+    }
+  }
+};
 
 template <typename Tin, typename T, int NB, int NW>
 __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T> p) {
@@ -211,16 +275,25 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
 #pragma unroll
   for (int q = 0; q < SQ; ++q) Sacc[q] = Mfma<T>::zero();
 
+  BlockStage<Tin, NB, NT> sa, sb;  // next diagonal block / next coupling block
   if (tid == 0) fail = 0;
-  load_block<Tin, T, NB, LD, NT, false>(A + (long long)start * NN, Abuf);
-  if (hasL) load_block<Tin, T, NB, LD, NT, true>(B + (long long)(start - 1) * NN, Rbuf + NB * LD);
+  sa.load(A + (long long)start * NN);
+  if (hasL) {
+    sb.load(B + (long long)(start - 1) * NN);
+    sb.template store<T, LD, true>(Rbuf + NB * LD);  // Z_1 = B_L^T
+  }
+  if (len > 1 || hasR) sb.load(B + (long long)start * NN);
+  sa.template store<T, LD, false>(Abuf);
   __syncthreads();
 
   for (int i = 0; i < len; ++i) {
     const int blk = start + i;
     const bool last = (i == len - 1);
     const bool hasC = !last || hasR;  // block couples to a later block
-    if (hasC) load_block<Tin, T, NB, LD, NT, false>(B + (long long)blk * NN, Rbuf);
+    if (hasC) sb.template store<T, LD, false>(Rbuf);  // B_blk (prefetched)
+    // prefetch the next block pair; it lands while this block is factored
+    if (!last) sa.load(A + (long long)(blk + 1) * NN);
+    if (i + 1 < len - 1 || (i + 1 == len - 1 && hasR)) sb.load(B + (long long)(blk + 1) * NN);
 
     potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
     if (tid == 0 && fail) {
@@ -241,8 +314,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
       if (task < ntile) {
         const int r = task % TT, c = c0 + task / TT;
         for (int t = 0; t <= r; ++t)
-          tile_mma<T, true, false, false>(X[q], Ubuf + tile_off(t, r, LD), LD,
-                                          Rbuf + tile_off(t, c, LD), LD, 16);
+          tile_mma<T, true, false, false, 16>(X[q], Ubuf + tile_off(t, r, LD), LD,
+                                          Rbuf + tile_off(t, c, LD), LD);
       }
     }
     __syncthreads();
@@ -262,7 +335,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
       const int r = idx % NB, c = idx / NB;
       if ((r >> 4) <= (c >> 4)) U[(long long)blk * NN + idx] = Ubuf[c * LD + r];
     }
-    if (!last) load_block<Tin, T, NB, LD, NT, false>(A + (long long)(blk + 1) * NN, Abuf);
+    if (!last) sa.template store<T, LD, false>(Abuf);  // A_{blk+1} (prefetched)
     __syncthreads();
 
     // S_L += W^T W (upper tiles, persistent per wave)
@@ -277,8 +350,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
             ++a;
           }
           const int b = a + s;
-          tile_mma<T, true, false, false>(Sacc[q], Rbuf + (NB + 16 * a) * LD, LD,
-                                          Rbuf + (NB + 16 * b) * LD, LD, NB);
+          tile_mma<T, true, false, false, NB>(Sacc[q], Rbuf + (NB + 16 * a) * LD, LD,
+                                          Rbuf + (NB + 16 * b) * LD, LD);
         }
       }
     }
@@ -292,8 +365,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
           if (hasL && q < ZQ) {
             const int a = task % TT, b = task / TT;
             Z[q] = Mfma<T>::zero();
-            tile_mma<T, true, false, true>(Z[q], Rbuf + 16 * a * LD, LD,
-                                           Rbuf + (NB + 16 * b) * LD, LD, NB);
+            tile_mma<T, true, false, true, NB>(Z[q], Rbuf + 16 * a * LD, LD,
+                                           Rbuf + (NB + 16 * b) * LD, LD);
           }
         } else if (task < TT * TT + NS) {
           int s = task - TT * TT, a = 0;
@@ -303,7 +376,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
           }
           const int b = a + s;
           acc_t acc = tile_load<T>(Abuf + tile_off(a, b, LD), LD);
-          tile_mma<T, true, false, true>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD, NB);
+          tile_mma<T, true, false, true, NB>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD);
           tile_store<T>(Abuf + tile_off(a, b, LD), LD, acc);
         }
       }
@@ -330,7 +403,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
           }
           const int b = a + s;
           acc_t acc = Mfma<T>::zero();
-          tile_mma<T, true, false, false>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD, NB);
+          tile_mma<T, true, false, false, NB>(acc, Rbuf + 16 * a * LD, LD, Rbuf + 16 * b * LD, LD);
           tile_store<T>(sr + tile_off(a, b, NB), NB, acc);
         }
         if (hasL) {
@@ -338,8 +411,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
           for (int task = w; task < TT * TT; task += NW) {
             const int a = task % TT, b = task / TT;
             acc_t acc = Mfma<T>::zero();
-            tile_mma<T, true, false, true>(acc, Rbuf + (NB + 16 * a) * LD, LD,
-                                           Rbuf + 16 * b * LD, LD, NB);
+            tile_mma<T, true, false, true, NB>(acc, Rbuf + (NB + 16 * a) * LD, LD,
+                                           Rbuf + 16 * b * LD, LD);
             tile_store<T>(bn + tile_off(a, b, NB), NB, acc);
           }
         }

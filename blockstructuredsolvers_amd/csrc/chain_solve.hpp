@@ -102,23 +102,23 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs<T>
     const long long blk = start + i;
     acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
     if (i > 0)
-      tile_mma<T, true, false, true>(acc, C + (blk - 1) * NN + 16 * w * NB, NB, Y, LDV, NB);
+      tile_mma<T, true, false, true, NB>(acc, C + (blk - 1) * NN + 16 * w * NB, NB, Y, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
     __syncthreads();
     acc = Mfma<T>::zero();
     for (int t = 0; t <= w; ++t)
-      tile_mma<T, true, false, false>(acc, U + blk * NN + 16 * w * NB + 16 * t, NB, R + 16 * t,
-                                      LDV, 16);
+      tile_mma<T, true, false, false, 16>(acc, U + blk * NN + 16 * w * NB + 16 * t, NB, R + 16 * t,
+                                      LDV);
     store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
     tile_store<T>(Y + 16 * w, LDV, acc);
     __syncthreads();
     if (sepL >= 0)
-      tile_mma<T, true, false, false>(accL, W + blk * NN + 16 * w * NB, NB, Y, LDV, NB);
+      tile_mma<T, true, false, false, NB>(accL, W + blk * NN + 16 * w * NB, NB, Y, LDV);
   }
   if (sepR >= 0) {
     const long long last = start + len - 1;
     acc_t acc = Mfma<T>::zero();
-    tile_mma<T, true, false, false>(acc, C + last * NN + 16 * w * NB, NB, Y, LDV, NB);
+    tile_mma<T, true, false, false, NB>(acc, C + last * NN + 16 * w * NB, NB, Y, LDV);
     store_rhs_tile<T, NB>(p.rR + sys * p.sysR + sepR * ND, w, nrhs, acc);
   }
   if (sepL >= 0) store_rhs_tile<T, NB>(p.rL + sys * p.sysR + sepL * ND, w, nrhs, accL);
@@ -160,14 +160,14 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs<T
     const long long blk = start + i;
     acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
     if (i < len - 1 || sepR >= 0)
-      tile_mma<T, false, false, true>(acc, C + blk * NN + 16 * w, NB, Xn, LDV, NB);
-    if (sepL >= 0) tile_mma<T, false, false, true>(acc, W + blk * NN + 16 * w, NB, XL, LDV, NB);
+      tile_mma<T, false, false, true, NB>(acc, C + blk * NN + 16 * w, NB, Xn, LDV);
+    if (sepL >= 0) tile_mma<T, false, false, true, NB>(acc, W + blk * NN + 16 * w, NB, XL, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
     __syncthreads();
     acc = Mfma<T>::zero();
     for (int t = w; t < TT; ++t)
-      tile_mma<T, false, false, false>(acc, U + blk * NN + 16 * t * NB + 16 * w, NB, R + 16 * t,
-                                       LDV, 16);
+      tile_mma<T, false, false, false, 16>(acc, U + blk * NN + 16 * t * NB + 16 * w, NB, R + 16 * t,
+                                       LDV);
     store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
     tile_store<T>(Xn + 16 * w, LDV, acc);
     __syncthreads();

@@ -62,27 +62,28 @@ __device__ inline float rsqrt_t<float>(float x) {
   return 1.0f / sqrtf(x);
 }
 
-// acc += op(X) * op(Y) for one 16x16 output tile, K a multiple of 16.
+// acc += op(X) * op(Y) for one 16x16 output tile, K a multiple of 16 (compile-time, so
+// the whole product is unrolled and every operand load issues before the MFMAs).
 //   TA: op(X) = X^T  -> P[i][k] = X[k][i] at xb[i*ldx + k]; else X[i][k] at xb[k*ldx + i]
 //   TB: op(Y) = Y^T  -> Q[k][j] = Y[j][k] at yb[k*ldy + j]; else Y[k][j] at yb[j*ldy + k]
 //   NEG: subtract instead of add (the A operand is negated).
 // xb/yb point at element (0,0) of the operands' 16-row/col panels; they may live in
 // LDS or global memory.
-template <typename T, bool TA, bool TB, bool NEG>
+template <typename T, bool TA, bool TB, bool NEG, int K>
 __device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
-                                int ldx, const T* __restrict__ yb, int ldy, int K) {
+                                int ldx, const T* __restrict__ yb, int ldy) {
+  static_assert(K % 16 == 0, "K must be a multiple of 16");
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  T a[K / 4], b[K / 4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int k = k0 + 4 * lk + s;
-      T a = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
-      T b = TB ? yb[k * ldy + lr] : yb[lr * ldy + k];
-      if (NEG) a = -a;
-      acc = Mfma<T>::mma(a, b, acc);
-    }
+  for (int q = 0; q < K / 4; ++q) {
+    const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
+    a[q] = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
+    b[q] = TB ? yb[k * ldy + lr] : yb[lr * ldy + k];
   }
+#pragma unroll
+  for (int q = 0; q < K / 4; ++q) acc = Mfma<T>::mma(NEG ? -a[q] : a[q], b[q], acc);
 }
 
 // Load / store a D-layout tile from/to a column-major buffer (element (0,0) at p).

@@ -69,12 +69,11 @@ __global__ void __launch_bounds__(4 * NB)
       dmax = nmax(dmax, fabs(acc[v]));
       xmax = nmax(xmax, fabs(X[1][lr * LDV + row]));
     }
-    tile_mma<double, false, false, true>(acc, As + blk * NN + 16 * w, NB, X[1], LDV, NB);
+    tile_mma<double, false, false, true, NB>(acc, As + blk * NN + 16 * w, NB, X[1], LDV);
     if (blk > 0)
-      tile_mma<double, true, false, true>(acc, Bs + (blk - 1) * NN + 16 * w * NB, NB, X[0], LDV,
-                                          NB);
+      tile_mma<double, true, false, true, NB>(acc, Bs + (blk - 1) * NN + 16 * w * NB, NB, X[0], LDV);
     if (blk + 1 < N)
-      tile_mma<double, false, false, true>(acc, Bs + blk * NN + 16 * w, NB, X[2], LDV, NB);
+      tile_mma<double, false, false, true, NB>(acc, Bs + blk * NN + 16 * w, NB, X[2], LDV);
     if (lr < nrhs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {

@@ -10,7 +10,12 @@ template <typename T, bool TA, bool TB, bool NEG>
 __global__ void __launch_bounds__(64) probe_kernel(const T* X, int ldx, const T* Y, int ldy,
                                                    T* D, int K) {
   typename bss::Mfma<T>::acc_t acc = bss::tile_load<T>(D, 16);
-  bss::tile_mma<T, TA, TB, NEG>(acc, X, ldx, Y, ldy, K);
+  switch (K) {
+    case 16: bss::tile_mma<T, TA, TB, NEG, 16>(acc, X, ldx, Y, ldy); break;
+    case 32: bss::tile_mma<T, TA, TB, NEG, 32>(acc, X, ldx, Y, ldy); break;
+    case 48: bss::tile_mma<T, TA, TB, NEG, 48>(acc, X, ldx, Y, ldy); break;
+    default: bss::tile_mma<T, TA, TB, NEG, 64>(acc, X, ldx, Y, ldy); break;
+  }
   bss::tile_store<T>(D, 16, acc);
 }
 
@@ -31,7 +36,7 @@ static int run(int ta, int tb, int neg, const T* X, int ldx, const T* Y, int ldy
 // success.  dtype: 0 float, 1 double.
 extern "C" int tile_probe(int dtype, int ta, int tb, int neg, const void* X, int ldx,
                           const void* Y, int ldy, void* D, int K) {
-  if (K <= 0 || K % 16) return -1;
+  if (K <= 0 || K % 16 || K > 64) return -1;
   if (dtype == 1)
     return run<double>(ta, tb, neg, (const double*)X, ldx, (const double*)Y, ldy, (double*)D, K);
   return run<float>(ta, tb, neg, (const float*)X, ldx, (const float*)Y, ldy, (float*)D, K);
