@@ -542,7 +542,6 @@ int bss_factorize(bss_plan_t p, void* stream) {
   if (p->mode == BSS_MODE_MIXED) {
     rc = factor_levels<double, float>(p, s);
     if (rc == BSS_OK) {
-      ProfScope ps(p, BSS_K_MNORM, 0, s);
       if (bss::mixed_after_factor(p->mx, (const double*)p->A, (const double*)p->B, p->n,
                                   p->nsys, s) != 0)
         rc = fail(BSS_E_HIP, "norm kernel: %s", bss::mixed_last_error());

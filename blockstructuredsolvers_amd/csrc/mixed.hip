@@ -11,29 +11,38 @@ namespace bss {
 namespace {
 
 thread_local std::string g_mixed_err;
-constexpr int kMaxWG = 1024;   // residual workgroups per system (partial-max slots)
-constexpr int kParts = 3 * 16; // per workgroup: rmax, xmax, dmax for up to 16 columns
+constexpr int kMaxWG = 1024;  // residual workgroups per system (partial-max slots)
+// per workgroup partials: [0,16) max|r_j|, [16,32) max|x_j|, [32,48) max|d_j|,
+// [48] max row sum of |M| (when requested)
+constexpr int kParts = 64;
+constexpr int kMnormSlot = 48;
 
 size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 
 // NaN-propagating max (fmax would drop a NaN residual and report convergence)
 __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
-// r = d - M x_new with x_new = x_old + dx over the blocks [wg*bpw, (wg+1)*bpw).
-// One wave per 16-row tile; the three block products use the same MFMA tile helper as
-// the solver, with A_i and B_i read from HBM straight into operand registers.
-template <int NB>
+// Residual pass over the blocks [wg*bpw, (wg+1)*bpw) of one system:
+//   x_new = x_old + dx (FP64; written for the workgroup's own blocks),
+//   r = d - (B_{i-1}^T x_{i-1} + A_i x_i + B_i x_{i+1}) in FP64 on MFMA, stored as FP32
+//   for the next correction solve, plus per-column max |r|, |x|, |d|.
+// INIT: x = 0, so r = d with no products (the first pass of a solve).
+// With `mnorm`, the row sums of |M| are accumulated from the operand values the
+// products already hold, so ||M||_inf costs no extra pass over A and B.
+// One wave per 16-row tile; a rolling three-block window of x_new lives in LDS and
+// each block's A/B operands are issued before the window update to hide HBM latency.
+template <int NB, bool INIT>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const double* __restrict__ A, const double* __restrict__ B,
                     const double* __restrict__ d, const double* __restrict__ xold,
                     const float* __restrict__ dx, double* __restrict__ xnew,
                     float* __restrict__ rout, double* __restrict__ part, int N, int nrhs,
-                    int bpw, int nwg) {
-  constexpr int TT = NB / 16, NT = 64 * TT, LDV = NB + 2;
+                    int bpw, int nwg, int mnorm) {
+  constexpr int TT = NB / 16, NT = 64 * TT, LDV = NB + 2, KQ = NB / 4;
   constexpr long long NN = (long long)NB * NB;
   typedef Mfma<double>::acc_t acc_t;
   __shared__ double X[3][16 * LDV];
-  __shared__ double red[3][4 * TT][16];
+  __shared__ double red[4][4 * TT][16];
   const int sys = blockIdx.y, wg = blockIdx.x, tid = threadIdx.x;
   const int w = wave_id(), l = lane_id(), lr = l & 15, lk = l >> 4;
   const long long ND = (long long)NB * nrhs;
@@ -45,35 +54,73 @@ __global__ void __launch_bounds__(4 * NB)
   double* xn = xnew + sys * (long long)N * ND;
   float* ro = rout + sys * (long long)N * ND;
   for (int e = tid; e < 3 * 16 * LDV; e += NT) (&X[0][0])[e] = 0.0;
-  double rmax = 0.0, xmax = 0.0, dmax = 0.0;
+  double rmax = 0.0, xmax = 0.0, dmax = 0.0, mmax = 0.0;
   const int b0 = wg * bpw, b1 = min(N, b0 + bpw);
-  for (int blk = b0; blk < b1; ++blk) {
-    __syncthreads();
-    for (int e = tid; e < 3 * (int)ND; e += NT) {
-      const int k = e / (int)ND, ee = e % (int)ND;
-      const long long j = (long long)blk - 1 + k;
+
+  // x_new of block j into its window slot (zero outside the system); own blocks are
+  // also written to xnew
+  auto stage = [&](int j) {
+    double* slot = X[((j % 3) + 3) % 3];
+    for (int e = tid; e < (int)ND; e += NT) {
       double v = 0.0;
-      if (j >= 0 && j < N) {
-        if (xo) v = xo[j * ND + ee];
-        if (dxs) v += (double)dxs[j * ND + ee];
+      if (!INIT && j >= 0 && j < N) {
+        if (xo) v = xo[(long long)j * ND + e];
+        if (dxs) v += (double)dxs[(long long)j * ND + e];
       }
-      X[k][(ee / NB) * LDV + (ee % NB)] = v;
-      if (k == 1) xn[(long long)blk * ND + ee] = v;
+      slot[(e / NB) * LDV + (e % NB)] = v;
+      if (j >= b0 && j < b1) xn[(long long)j * ND + e] = v;
     }
-    __syncthreads();
+  };
+
+  for (int blk = b0; blk < b1; ++blk) {
+    // operands of this block row, issued first
+    double aA[KQ], aBt[KQ], aB[KQ];
+    if (!INIT) {
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) {
+        const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
+        aA[q] = As[blk * NN + (long long)k * NB + 16 * w + lr];
+        aBt[q] = blk > 0 ? Bs[(blk - 1) * NN + (long long)(16 * w + lr) * NB + k] : 0.0;
+        aB[q] = blk + 1 < N ? Bs[blk * NN + (long long)k * NB + 16 * w + lr] : 0.0;
+      }
+    }
     acc_t acc;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = 16 * w + Mfma<double>::row(lk, v);
       acc[v] = (lr < nrhs) ? ds[(long long)blk * ND + lr * NB + row] : 0.0;
       dmax = nmax(dmax, fabs(acc[v]));
-      xmax = nmax(xmax, fabs(X[1][lr * LDV + row]));
     }
-    tile_mma<double, false, false, true, NB>(acc, As + blk * NN + 16 * w, NB, X[1], LDV);
-    if (blk > 0)
-      tile_mma<double, true, false, true, NB>(acc, Bs + (blk - 1) * NN + 16 * w * NB, NB, X[0], LDV);
-    if (blk + 1 < N)
-      tile_mma<double, false, false, true, NB>(acc, Bs + blk * NN + 16 * w, NB, X[2], LDV);
+    __syncthreads();  // previous block's window reads are done
+    if (blk == b0) {
+      stage(blk - 1);
+      stage(blk);
+    }
+    stage(blk + 1);
+    __syncthreads();
+    const double* Xm = X[(((blk - 1) % 3) + 3) % 3];
+    const double* X0 = X[blk % 3];
+    const double* Xp = X[(blk + 1) % 3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      xmax = nmax(xmax, fabs(X0[lr * LDV + 16 * w + Mfma<double>::row(lk, v)]));
+    if (!INIT) {
+      double rs = 0.0;
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) {
+        const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
+        acc = Mfma<double>::mma(-aA[q], X0[lr * LDV + k], acc);
+        acc = Mfma<double>::mma(-aBt[q], Xm[lr * LDV + k], acc);
+        acc = Mfma<double>::mma(-aB[q], Xp[lr * LDV + k], acc);
+        if (mnorm) rs += fabs(aA[q]) + fabs(aBt[q]) + fabs(aB[q]);
+      }
+      if (mnorm) {
+        // the four lanes sharing lr hold the four k-slices of row 16w + lr
+        rs += __shfl_xor(rs, 16);
+        rs += __shfl_xor(rs, 32);
+        mmax = nmax(mmax, rs);
+      }
+    }
     if (lr < nrhs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -85,20 +132,38 @@ __global__ void __launch_bounds__(4 * NB)
   red[0][4 * w + lk][lr] = rmax;
   red[1][4 * w + lk][lr] = xmax;
   red[2][4 * w + lk][lr] = dmax;
+  red[3][4 * w + lk][lr] = mmax;
   __syncthreads();
   if (tid < kParts) {
     const int q = tid / 16, c = tid % 16;
     double m = 0.0;
-    for (int k = 0; k < 4 * TT; ++k) m = nmax(m, red[q][k][c]);
-    part[((long long)sys * nwg + wg) * kParts + tid] = m;
+    if (q < 3) {
+      for (int k = 0; k < 4 * TT; ++k) m = nmax(m, red[q][k][c]);
+    } else if (c == 0) {
+      for (int k = 0; k < 4 * TT; ++k)
+        for (int cc = 0; cc < 16; ++cc) m = nmax(m, red[3][k][cc]);
+    }
+    if (q < 3 || c == 0) part[((long long)sys * nwg + wg) * kParts + (q < 3 ? tid : kMnormSlot)] = m;
   }
 }
 
-// combine partial maxima -> berr per (system, column); `init` records ||d||.
-__global__ void berr_kernel(const double* __restrict__ part, const double* __restrict__ mnorm,
+// Combine the partial maxima: berr per (system, column).  `init` records ||d_j||;
+// `mnorm` first reduces ||M||_inf from the same pass.
+__global__ void berr_kernel(const double* __restrict__ part, double* __restrict__ mnorm,
                             double* __restrict__ dnorm, double* __restrict__ berr, int nwg,
-                            int nrhs, int init) {
+                            int nrhs, int init, int want_mnorm) {
   const int sys = blockIdx.x, c = threadIdx.x;
+  __shared__ double mn;
+  if (c == 0) {
+    double m = mnorm[sys];
+    if (want_mnorm) {
+      m = 0.0;
+      for (int g = 0; g < nwg; ++g) m = nmax(m, part[((long long)sys * nwg + g) * kParts + kMnormSlot]);
+      mnorm[sys] = m;
+    }
+    mn = m;
+  }
+  __syncthreads();
   if (c >= nrhs) return;
   double r = 0.0, x = 0.0, dd = 0.0;
   for (int g = 0; g < nwg; ++g) {
@@ -108,53 +173,8 @@ __global__ void berr_kernel(const double* __restrict__ part, const double* __res
     dd = nmax(dd, pp[32 + c]);
   }
   if (init) dnorm[sys * 16 + c] = dd;
-  const double den = mnorm[sys] * x + dnorm[sys * 16 + c];
+  const double den = mn * x + dnorm[sys * 16 + c];
   berr[sys * nrhs + c] = den > 0.0 ? r / den : r;
-}
-
-// ||M||_inf: max over block rows of sum_c |A_i| + |B_{i-1}^T| + |B_i|, per system.
-template <int NB>
-__global__ void __launch_bounds__(4 * NB) mnorm_kernel(const double* __restrict__ A,
-                                                       const double* __restrict__ B,
-                                                       double* __restrict__ part, int N,
-                                                       int bpw, int nwg) {
-  constexpr long long NN = (long long)NB * NB;
-  __shared__ double rs[4][NB];
-  const int sys = blockIdx.y, wg = blockIdx.x, tid = threadIdx.x;
-  const int r = tid % NB, q = tid / NB;
-  const double* As = A + sys * (long long)N * NN;
-  const double* Bs = B + sys * (long long)(N > 0 ? N - 1 : 0) * NN;
-  double best = 0.0;
-  const int b0 = wg * bpw, b1 = min(N, b0 + bpw);
-  for (int blk = b0; blk < b1; ++blk) {
-    double s = 0.0;
-    for (int c = q; c < NB; c += 4) {
-      s += fabs(As[blk * NN + c * NB + r]);
-      if (blk > 0) s += fabs(Bs[(blk - 1) * NN + r * NB + c]);
-      if (blk + 1 < N) s += fabs(Bs[blk * NN + c * NB + r]);
-    }
-    __syncthreads();
-    rs[q][r] = s;
-    __syncthreads();
-    if (q == 0) best = nmax(best, rs[0][r] + rs[1][r] + rs[2][r] + rs[3][r]);
-  }
-  __syncthreads();
-  if (q == 0) rs[0][r] = best;
-  __syncthreads();
-  if (tid == 0) {
-    double m = 0.0;
-    for (int k = 0; k < NB; ++k) m = nmax(m, rs[0][k]);
-    part[((long long)sys * nwg + wg) * kParts] = m;
-  }
-}
-
-__global__ void mnorm_reduce_kernel(const double* __restrict__ part, double* __restrict__ mnorm,
-                                    int nwg) {
-  const int sys = blockIdx.x;
-  if (threadIdx.x != 0) return;
-  double m = 0.0;
-  for (int g = 0; g < nwg; ++g) m = nmax(m, part[((long long)sys * nwg + g) * kParts]);
-  mnorm[sys] = m;
 }
 
 void geometry(int N, int* nwg, int* bpw) {
@@ -166,26 +186,30 @@ void geometry(int N, int* nwg, int* bpw) {
 template <int NB>
 hipError_t launch_residual(const MixedState& m, const double* A, const double* B,
                            const double* d, const double* xold, const float* dx, double* xnew,
-                           float* rout, hipStream_t s) {
-  hipLaunchKernelGGL((residual_kernel<NB>), dim3(m.nwg, m.nsys), dim3(4 * NB), 0, s, A, B, d,
-                     xold, dx, xnew, rout, m.part, m.N, m.nrhs, m.bpw, m.nwg);
+                           float* rout, bool init, int want_mnorm, hipStream_t s) {
+  if (init)
+    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nwg, m.nsys), dim3(4 * NB), 0, s, A,
+                       B, d, xold, dx, xnew, rout, m.part, m.N, m.nrhs, m.bpw, m.nwg, 0);
+  else
+    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nwg, m.nsys), dim3(4 * NB), 0, s, A,
+                       B, d, xold, dx, xnew, rout, m.part, m.N, m.nrhs, m.bpw, m.nwg, want_mnorm);
   return hipGetLastError();
 }
 
 hipError_t residual(const MixedState& m, const double* A, const double* B, const double* d,
                     const double* xold, const float* dx, double* xnew, float* rout, int init,
-                    hipStream_t s) {
+                    int want_mnorm, hipStream_t s) {
   hipError_t e;
   switch (m.n) {
-    case 16: e = launch_residual<16>(m, A, B, d, xold, dx, xnew, rout, s); break;
-    case 32: e = launch_residual<32>(m, A, B, d, xold, dx, xnew, rout, s); break;
-    case 48: e = launch_residual<48>(m, A, B, d, xold, dx, xnew, rout, s); break;
-    case 64: e = launch_residual<64>(m, A, B, d, xold, dx, xnew, rout, s); break;
+    case 16: e = launch_residual<16>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
+    case 32: e = launch_residual<32>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
+    case 48: e = launch_residual<48>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
+    case 64: e = launch_residual<64>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(64), 0, s, m.part, m.mnorm, m.dnorm, m.berr,
-                     m.nwg, m.nrhs, init);
+                     m.nwg, m.nrhs, init, want_mnorm);
   return hipGetLastError();
 }
 
@@ -232,22 +256,12 @@ void MixedState::release() {
 
 int mixed_after_factor(MixedState& m, const double* A, const double* B, int n, int nsys,
                        hipStream_t s) {
-  switch (n) {
-#define BSS_MN(NB)                                                                          \
-  case NB:                                                                                  \
-    hipLaunchKernelGGL((mnorm_kernel<NB>), dim3(m.nwg, nsys), dim3(4 * NB), 0, s, A, B,      \
-                       m.part, m.N, m.bpw, m.nwg);                                          \
-    break;
-    BSS_MN(16) BSS_MN(32) BSS_MN(48) BSS_MN(64)
-#undef BSS_MN
-    default: g_mixed_err = "unsupported n"; return -1;
-  }
-  hipLaunchKernelGGL(mnorm_reduce_kernel, dim3(nsys), dim3(64), 0, s, m.part, m.mnorm, m.nwg);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    g_mixed_err = hipGetErrorString(e);
-    return -1;
-  }
+  (void)A;
+  (void)B;
+  (void)n;
+  (void)nsys;
+  (void)s;
+  m.mnorm_stale = true;  // recomputed by the next solve's first residual pass
   return 0;
 }
 
@@ -256,9 +270,10 @@ constexpr int kKindResidual = 6;  // BSS_K_RESIDUAL in bss.h
 
 hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, const double* A,
                          const double* B, const double* d, const double* xold,
-                         const float* dx, double* xnew, float* rout, int init, hipStream_t s) {
+                         const float* dx, double* xnew, float* rout, int init, int want_mnorm,
+                         hipStream_t s) {
   void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindResidual, s) : nullptr;
-  hipError_t e = residual(m, A, B, d, xold, dx, xnew, rout, init, s);
+  hipError_t e = residual(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s);
   if (scope) h.prof_end(scope);
   return e;
 }
@@ -271,7 +286,7 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
   (void)nrhs;
   float* v[2] = {v0, m.corr};
   int cur = 0, xa = 0, it = 0;
-  hipError_t e = residual_prof(m, h, A, B, d, nullptr, nullptr, m.x[0], v[0], 1, s);
+  hipError_t e = residual_prof(m, h, A, B, d, nullptr, nullptr, m.x[0], v[0], 1, 0, s);
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;
@@ -281,7 +296,9 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
       g_mixed_err = "FP32 solve failed";
       return -1;
     }
-    e = residual_prof(m, h, A, B, d, m.x[xa], v[cur], m.x[1 - xa], v[1 - cur], 0, s);
+    const int want_mnorm = m.mnorm_stale ? 1 : 0;
+    e = residual_prof(m, h, A, B, d, m.x[xa], v[cur], m.x[1 - xa], v[1 - cur], 0, want_mnorm, s);
+    m.mnorm_stale = false;
     if (e == hipSuccess)
       e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * nsys * m.nrhs,
                          hipMemcpyDeviceToHost, s);

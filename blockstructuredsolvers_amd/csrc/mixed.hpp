@@ -24,6 +24,7 @@ struct MixedState {
   double* dnorm = nullptr;            // ||d_j||_inf per system and column
   double* berr = nullptr;             // backward error per system and column
   double* host_berr = nullptr;        // pinned mirror of berr
+  bool mnorm_stale = true;            // ||M|| must be recomputed (new factorisation)
   std::vector<double> last_berr;
 
   static size_t bytes(int N, int n, int nrhs, int nsys);
@@ -40,7 +41,8 @@ struct MixedSolveHooks {
   void (*prof_end)(void* scope) = nullptr;
 };
 
-// ||M||_inf of every system (after the factorisation; A and B are not modified).
+// Marks ||M||_inf stale after a factorisation; the next solve's first residual pass
+// recomputes it from the operands it reads anyway.
 int mixed_after_factor(MixedState& m, const double* A, const double* B, int n, int nsys,
                        hipStream_t s);
 

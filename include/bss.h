@@ -90,7 +90,7 @@ enum bss_kernel_kind {
   BSS_K_BACKWARD = 4,      /* backward substitution sweep */
   BSS_K_SCATTER = 5,       /* separator solution scatter */
   BSS_K_RESIDUAL = 6,      /* FP64 residual + update (mixed mode) */
-  BSS_K_MNORM = 7,         /* ||M||_inf (mixed mode) */
+  BSS_K_MNORM = 7,         /* reserved: ||M||_inf is folded into the first residual */
   BSS_K_COUNT = 8
 };
 

@@ -128,3 +128,66 @@ def test_non_spd_reports_info():
     assert info > 0
     ref_info = oracle.factorize(A.copy(order="F"), sysm.B.copy(order="F"))
     assert ref_info > 0
+
+
+def run_gpu_batch(systems, T=torch.float64, **kw):
+    N, n, nrhs = systems[0].N, systems[0].n, systems[0].nrhs
+    data = initialize_batched(N, n, T, "cuda", nrhs=nrhs, nsys=len(systems), **kw)
+    for s, sysm in enumerate(systems):
+        data.A_tensor[:, :, :N, s].copy_(torch.from_numpy(sysm.A))
+        data.B_tensor[:, :, :N - 1, s].copy_(torch.from_numpy(sysm.B))
+        data.d_tensor[:, :, :N, s].copy_(torch.from_numpy(sysm.d))
+    factorize(data)
+    solve(data)
+    torch.cuda.synchronize()
+    return [data.d_tensor[:, :, :N, s].double().cpu().numpy() for s in range(len(systems))], data
+
+
+@pytest.mark.parametrize("n,nrhs", [(32, 1), (64, 1), (64, 16), (16, 3)])
+def test_mixed_refined_backward_error(n, nrhs):
+    """FP32 MFMA factorisation + FP64 refinement: normwise backward error <= 1e-12 as
+    measured by the FP64 oracle, agreeing with the device's own measurement."""
+    sysm = workloads.spd_system(300, n, nrhs=nrhs, seed=100 + n)
+    x, data = run_gpu(sysm, mode="mixed")
+    berr = oracle.backward_error(sysm.A, sysm.B, x, sysm.d)
+    assert berr.max() <= 1e-12, berr
+    dev = data.backward_error()[0]
+    assert dev.max() <= 1e-12
+    assert np.allclose(dev, berr, rtol=0.5, atol=1e-16)
+    assert 1 <= data.iterations <= 10
+    xo, _ = oracle.solve_system(sysm.A, sysm.B, sysm.d)
+    assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+    # inputs untouched in mixed mode
+    assert np.array_equal(data.A_tensor[:, :, :sysm.N].cpu().numpy(), sysm.A)
+
+
+def test_mixed_repeated_solves_and_schedules():
+    sysm = workloads.spd_system(257, 32, nrhs=2, seed=17)
+    for schedule in ("auto", "reference"):
+        x, data = run_gpu(sysm, mode="mixed", schedule=schedule)
+        assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
+        rng = np.random.default_rng(3)
+        d2 = np.asfortranarray(rng.uniform(size=sysm.d.shape))
+        data.d_tensor[:, :, :sysm.N].copy_(torch.from_numpy(d2))
+        solve(data)
+        torch.cuda.synchronize()
+        x2 = data.d_tensor[:, :, :sysm.N].cpu().numpy()
+        assert oracle.backward_error(sysm.A, sysm.B, x2, d2).max() <= 1e-12
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_batch_equals_alone(mode):
+    """A system's arithmetic does not depend on its co-residents: solving it inside a
+    batch of three gives bitwise the same answer as solving it alone."""
+    systems = [workloads.spd_system(400, 32, nrhs=1, seed=200 + k) for k in range(3)]
+    xs, data = run_gpu_batch(systems, mode=mode)
+    for k, sysm in enumerate(systems):
+        xa, _ = run_gpu(sysm, mode=mode)
+        if mode == "plain":
+            assert np.array_equal(xs[k], xa), k
+        else:
+            # the iteration count is shared by the batch (stop when all converge), so a
+            # system may get one more refinement step than alone: equal to the tolerance
+            assert np.abs(xs[k] - xa).max() <= 1e-12 * np.abs(xa).max()
+        assert oracle.backward_error(sysm.A, sysm.B, xs[k], sysm.d).max() <= (
+            1e-13 if mode == "plain" else 1e-12)
