@@ -81,7 +81,8 @@ def run_path(mode, systems, args, torch, bss):
     data = bss.initialize_batched(N, n, torch.float64, "cuda", nrhs=nrhs, nsys=nsys, mode=mode,
                                   schedule=args.schedule, chain_len=args.chain_len,
                                   base_max=args.base_max, preserve_input=(mode == "plain"),
-                                  tol=args.tol if mode == "mixed" else None)
+                                  tol=args.tol if mode == "mixed" else None,
+                                  chain_len_levels=args.chain_lens, factor_waves=args.factor_waves)
     upload(data, systems, torch)
     d0 = data._d.clone()
     torch.cuda.synchronize()
@@ -221,6 +222,9 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "reference"])
     ap.add_argument("--chain-len", type=int, default=None)
     ap.add_argument("--base-max", type=int, default=None)
+    ap.add_argument("--chain-lens", type=lambda v: [int(x) for x in v.split(",")], default=None,
+                    help="per-level chain lengths, top level first")
+    ap.add_argument("--factor-waves", type=int, default=None)
     ap.add_argument("--lat-solves", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)

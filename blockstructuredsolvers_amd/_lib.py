@@ -31,6 +31,8 @@ BSS_OPT_PIVOT_TOL = 2
 BSS_OPT_CHAIN_LEN = 3
 BSS_OPT_BASE_MAX = 4
 BSS_OPT_PRESERVE_INPUT = 5
+BSS_OPT_FACTOR_WAVES = 6
+BSS_OPT_CHAIN_LEN_LEVEL = 32
 
 KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,
                 "backward": 4, "scatter": 5, "residual": 6, "mnorm": 7}

@@ -75,8 +75,10 @@ struct bss_plan {
   bool built = false;
   double tol = 1e-12, pivot_tol = 1e-5;
   int max_iters = 10;
-  std::vector<int> chain_len;
+  std::vector<int> chain_len = std::vector<int>(16, 0);  // per-level overrides
+  int chain_len_default = 3;
   int base_max = 8;
+  int factor_waves = 4;
   bool preserve = false;  // plain mode: factors to workspace, A/B untouched
   // mixed-precision state
   bss::MixedState mx;
@@ -151,7 +153,8 @@ int build(bss_plan* p, bool alloc = true) {
     shapes = bss::reference_schedule_levels(p->N, &n_use);
     p->N_use = n_use;
   } else {
-    shapes = bss::auto_schedule(p->N, p->chain_len, std::max(1, p->base_max));
+    shapes = bss::auto_schedule(p->N, p->chain_len, p->chain_len_default,
+                                std::max(1, p->base_max));
     p->N_use = p->N;
   }
   const size_t nn = (size_t)p->n * p->n, nd = (size_t)p->n * p->nrhs;
@@ -295,9 +298,9 @@ int factor_levels(bss_plan* p, hipStream_t s) {
     {
       ProfScope ps(p, BSS_K_FACTOR, (int)l, s);
       if (l == 0)
-        e = bss::launch_factor<Tin, T>(nb, a, L.nch, p->nsys, s);
+        e = bss::launch_factor<Tin, T>(nb, p->factor_waves, a, L.nch, p->nsys, s);
       else
-        e = bss::launch_factor<T, T>(nb, a, L.nch, p->nsys, s);
+        e = bss::launch_factor<T, T>(nb, p->factor_waves, a, L.nch, p->nsys, s);
     }
     if (e != hipSuccess) return fail(BSS_E_HIP, "factor level %zu launch: %s", l, hipGetErrorString(e));
     if (Nx) {
@@ -458,7 +461,11 @@ int bss_set_option(bss_plan_t p, int option, double value) {
       return BSS_OK;
     case BSS_OPT_CHAIN_LEN:
       if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "chain length must be >= 1");
-      p->chain_len.assign(1, (int)value);
+      p->chain_len_default = (int)value;
+      return BSS_OK;
+    case BSS_OPT_FACTOR_WAVES:
+      if (value != 4 && value != 8) return fail(BSS_E_ARG, "factor waves must be 4 or 8");
+      p->factor_waves = (int)value;
       return BSS_OK;
     case BSS_OPT_BASE_MAX:
       if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "base_max must be >= 1");
@@ -469,6 +476,12 @@ int bss_set_option(bss_plan_t p, int option, double value) {
       p->preserve = value != 0.0;
       return BSS_OK;
     default:
+      if (option >= BSS_OPT_CHAIN_LEN_LEVEL && option < BSS_OPT_CHAIN_LEN_LEVEL + 16) {
+        if (p->built) return fail(BSS_E_STATE, "schedule options must precede the first bind");
+        if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "chain length must be >= 1");
+        p->chain_len[option - BSS_OPT_CHAIN_LEN_LEVEL] = (int)value;
+        return BSS_OK;
+      }
       return fail(BSS_E_ARG, "unknown option %d", option);
   }
 }

@@ -2,7 +2,7 @@
 // mixed-precision path below its FP64 top-level input).
 #include "kernels_impl.hpp"
 namespace bss {
-template hipError_t launch_factor<float, float>(int, const FactorLevelArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_factor<float, float>(int, int, const FactorLevelArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_forward<float>(int, const SolveLevelArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_backward<float>(int, const SolveLevelArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_assemble_diag<float, float>(const float*, const float*, const float*, const int*, float*, int, int, int, long long, long long, long long, hipStream_t);

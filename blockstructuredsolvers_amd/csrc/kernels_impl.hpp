@@ -7,23 +7,24 @@
 
 namespace bss {
 
-template <typename Tin, typename T, int NB>
+template <typename Tin, typename T, int NB, int NW>
 static hipError_t factor_nb(const FactorLevelArgs<T>& a, int nchains, int nsys, hipStream_t s) {
-  constexpr int NW = NB / 16;
   hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW>), dim3(nchains, nsys), dim3(64 * NW),
                      0, s, a);
   return hipGetLastError();
 }
 
 template <typename Tin, typename T>
-hipError_t launch_factor(int nb, const FactorLevelArgs<T>& a, int nchains, int nsys,
+hipError_t launch_factor(int nb, int waves, const FactorLevelArgs<T>& a, int nchains, int nsys,
                          hipStream_t s) {
   if (nchains <= 0) return hipSuccess;
   switch (nb) {
-    case 16: return factor_nb<Tin, T, 16>(a, nchains, nsys, s);
-    case 32: return factor_nb<Tin, T, 32>(a, nchains, nsys, s);
-    case 48: return factor_nb<Tin, T, 48>(a, nchains, nsys, s);
-    case 64: return factor_nb<Tin, T, 64>(a, nchains, nsys, s);
+    case 16: return factor_nb<Tin, T, 16, 1>(a, nchains, nsys, s);
+    case 32: return factor_nb<Tin, T, 32, 2>(a, nchains, nsys, s);
+    case 48: return factor_nb<Tin, T, 48, 3>(a, nchains, nsys, s);
+    case 64:
+      if (waves == 8) return factor_nb<Tin, T, 64, 8>(a, nchains, nsys, s);
+      return factor_nb<Tin, T, 64, 4>(a, nchains, nsys, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -10,8 +10,9 @@
 namespace bss {
 
 // returns hipErrorInvalidValue for an unsupported block size
+// `waves`: workgroup size in waves for nb = 64 (4 or 8); other sizes use nb/16
 template <typename Tin, typename T>
-hipError_t launch_factor(int nb, const FactorLevelArgs<T>& a, int nchains, int nsys,
+hipError_t launch_factor(int nb, int waves, const FactorLevelArgs<T>& a, int nchains, int nsys,
                          hipStream_t s);
 template <typename T>
 hipError_t launch_forward(int nb, const SolveLevelArgs<T>& a, int nchains, int nsys,

@@ -148,32 +148,50 @@ __global__ void __launch_bounds__(4 * NB)
 }
 
 // Combine the partial maxima: berr per (system, column).  `init` records ||d_j||;
-// `mnorm` first reduces ||M||_inf from the same pass.
-__global__ void berr_kernel(const double* __restrict__ part, double* __restrict__ mnorm,
-                            double* __restrict__ dnorm, double* __restrict__ berr, int nwg,
-                            int nrhs, int init, int want_mnorm) {
-  const int sys = blockIdx.x, c = threadIdx.x;
-  __shared__ double mn;
-  if (c == 0) {
-    double m = mnorm[sys];
-    if (want_mnorm) {
-      m = 0.0;
-      for (int g = 0; g < nwg; ++g) m = nmax(m, part[((long long)sys * nwg + g) * kParts + kMnormSlot]);
-      mnorm[sys] = m;
-    }
-    mn = m;
+// `mnorm` first reduces ||M||_inf from the same pass.  One 256-thread workgroup per
+// system: each thread folds every 256th partial record, then wave shuffles and LDS
+// finish the reduction.
+constexpr int kRedThreads = 256;
+constexpr int kSlots = 49;  // 3 x 16 column maxima + the |M| row-sum maximum
+
+__global__ void __launch_bounds__(kRedThreads)
+    berr_kernel(const double* __restrict__ part, double* __restrict__ mnorm,
+                double* __restrict__ dnorm, double* __restrict__ berr, int nwg, int nrhs,
+                int init, int want_mnorm) {
+  __shared__ double red[kRedThreads / 64][kSlots];
+  const int sys = blockIdx.x, tid = threadIdx.x;
+  double m[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) m[k] = 0.0;
+  for (int g = tid; g < nwg; g += kRedThreads) {
+    const double* pp = part + ((long long)sys * nwg + g) * kParts;
+#pragma unroll
+    for (int k = 0; k < 48; ++k) m[k] = nmax(m[k], pp[k]);
+    m[48] = nmax(m[48], pp[kMnormSlot]);
+  }
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m[k] = nmax(m[k], __shfl_xor(m[k], off));
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) red[tid >> 6][k] = m[k];
   }
   __syncthreads();
+  if (tid >= 16) return;
+  auto fold = [&](int k) {
+    double v = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) v = nmax(v, red[w][k]);
+    return v;
+  };
+  const double mn = want_mnorm ? fold(48) : mnorm[sys];
+  if (want_mnorm && tid == 0) mnorm[sys] = mn;
+  const int c = tid;
   if (c >= nrhs) return;
-  double r = 0.0, x = 0.0, dd = 0.0;
-  for (int g = 0; g < nwg; ++g) {
-    const double* pp = part + ((long long)sys * nwg + g) * kParts;
-    r = nmax(r, pp[c]);
-    x = nmax(x, pp[16 + c]);
-    dd = nmax(dd, pp[32 + c]);
-  }
+  const double r = fold(c), x = fold(16 + c), dd = fold(32 + c);
   if (init) dnorm[sys * 16 + c] = dd;
-  const double den = mn * x + dnorm[sys * 16 + c];
+  const double den = mn * x + (init ? dd : dnorm[sys * 16 + c]);
   berr[sys * nrhs + c] = den > 0.0 ? r / den : r;
 }
 
@@ -208,8 +226,8 @@ hipError_t residual(const MixedState& m, const double* A, const double* B, const
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(64), 0, s, m.part, m.mnorm, m.dnorm, m.berr,
-                     m.nwg, m.nrhs, init, want_mnorm);
+  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(kRedThreads), 0, s, m.part, m.mnorm, m.dnorm,
+                     m.berr, m.nwg, m.nrhs, init, want_mnorm);
   return hipGetLastError();
 }
 

@@ -117,13 +117,14 @@ inline LevelShape split_level(int N, int m) {
 }
 
 // Full schedule: levels top first, the last one a single chain.
+// chain_len[k] > 0 fixes level k's chain length; other levels use `m_default`.
 inline std::vector<LevelShape> auto_schedule(int N, const std::vector<int>& chain_len,
-                                             int base_max) {
+                                             int m_default, int base_max) {
   std::vector<LevelShape> out;
   int cur = N;
   size_t li = 0;
   while (cur > base_max && cur >= 3) {
-    const int m = chain_len.empty() ? 3 : chain_len[std::min(li, chain_len.size() - 1)];
+    const int m = (li < chain_len.size() && chain_len[li] > 0) ? chain_len[li] : m_default;
     LevelShape L = split_level(cur, std::max(1, m));
     const int P = (int)L.seppos.size();
     out.push_back(std::move(L));

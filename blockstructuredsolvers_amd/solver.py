@@ -53,7 +53,9 @@ class BlockTriDiagData_batched:
     def __init__(self, N: int, n: int, T, device, nrhs: int, nsys: int, mode: str,
                  schedule: str, chain_len: Optional[int], base_max: Optional[int],
                  preserve_input: bool = False, tol: Optional[float] = None,
-                 max_iters: Optional[int] = None):
+                 max_iters: Optional[int] = None,
+                 chain_len_levels: Optional[List[int]] = None,
+                 factor_waves: Optional[int] = None):
         self.n_user = N
         self.n = n
         self.nrhs = nrhs
@@ -73,6 +75,11 @@ class BlockTriDiagData_batched:
             L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
         if base_max is not None:
             L.call("bss_set_option", plan, L.BSS_OPT_BASE_MAX, float(base_max))
+        if chain_len_levels is not None:
+            for k, m in enumerate(list(chain_len_levels)[:16]):
+                L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN_LEVEL + k, float(m))
+        if factor_waves is not None:
+            L.call("bss_set_option", plan, L.BSS_OPT_FACTOR_WAVES, float(factor_waves))
         if preserve_input:
             L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
         if tol is not None:
@@ -167,7 +174,10 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
                        chain_len: Optional[int] = None,
                        base_max: Optional[int] = None, preserve_input: bool = False,
                        tol: Optional[float] = None,
-                       max_iters: Optional[int] = None) -> BlockTriDiagData_batched:
+                       max_iters: Optional[int] = None,
+                       chain_len_levels: Optional[List[int]] = None,
+                       factor_waves: Optional[int] = None,
+                       tuned: bool = True) -> BlockTriDiagData_batched:
     """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
 
     ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
@@ -176,14 +186,26 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     (FP32 factor + FP64 refinement, T must be Float64) and ``schedule="reference"``
     (the reference's level schedule and identity padding to N_use) or ``"auto"``;
     ``preserve_input`` keeps A and B unchanged in plain mode (factors go to the plan's
-    workspace); ``tol``/``max_iters`` control mixed-mode refinement."""
+    workspace); ``tol``/``max_iters`` control mixed-mode refinement; ``chain_len``,
+    ``chain_len_levels``, ``base_max`` and ``factor_waves`` fix the auto schedule,
+    otherwise (``tuned=True``) it comes from the autotuner cache when one exists."""
     if not torch.cuda.is_available():
         raise RuntimeError("initialize_batched needs a HIP device; none is visible")
     device = torch.device(M) if not isinstance(M, torch.device) else M
     if device.type != "cuda":
         raise ValueError(f"M must name a GPU device, got {M!r}")
+    if (tuned and schedule == "auto" and chain_len is None and base_max is None
+            and chain_len_levels is None and factor_waves is None):
+        # schedule from the autotuner cache (keyed by N, n, type, mode -- never by nsys)
+        from .autotune import lookup
+        cfg = lookup(N, n, T, mode)
+        if cfg:
+            chain_len_levels = cfg.get("chain_len_levels")
+            base_max = cfg.get("base_max")
+            factor_waves = cfg.get("factor_waves")
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
-                                    base_max, preserve_input, tol, max_iters)
+                                    base_max, preserve_input, tol, max_iters,
+                                    chain_len_levels, factor_waves)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:

@@ -76,9 +76,12 @@ enum bss_option {
                                re-factored in FP64 */
   BSS_OPT_CHAIN_LEN = 3,    /* auto schedule: chain length for every level */
   BSS_OPT_BASE_MAX = 4,     /* auto schedule: largest single-chain base level */
-  BSS_OPT_PRESERVE_INPUT = 5 /* plain mode: 1 = keep A and B unchanged, factors go to
+  BSS_OPT_PRESERVE_INPUT = 5, /* plain mode: 1 = keep A and B unchanged, factors go to
                                the plan's workspace (default 0: overwrite, as the
                                reference does) */
+  BSS_OPT_FACTOR_WAVES = 6,  /* waves per factor workgroup at n = 64: 4 (default) or 8 */
+  BSS_OPT_CHAIN_LEN_LEVEL = 32 /* + level k (< 16): chain length of level k only
+                                  (auto schedule); overrides BSS_OPT_CHAIN_LEN there */
 };
 
 /* Kernel kinds for the profiler (bss_profile_read). */
