@@ -94,55 +94,55 @@ __device__ inline float rsqrt_nr(float p) {
   return y * (1.5f - 0.5f * p * y * y);
 }
 
-// One elimination step J of the 16x16 tile (see diag_chol_inv16).
+// One elimination step J of the 16x16 tile (see diag_chol_inv16).  Lane c (of every
+// 16-lane row; the four rows compute the same thing) holds column c: a[r] = A[r][c],
+// e[r] = E[r][c].  Everything the step needs is in the lane's own registers or in
+// lane J of the same row, so the serial path uses only readlane/DPP and VALU ops.
 template <typename T, int J>
-__device__ inline void diag_step(T (&a)[4], T (&e)[4], bool& ok, int lr, int lk) {
-  constexpr int sg = J >> 2, sv = J & 3;
-  const T p = read_lane(a[sv], J + 16 * sg);  // pivot A[J][J]
+__device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&e)[16], bool& ok) {
+  // keep steps apart in the schedule: interleaving them buys no latency (each step
+  // depends on the previous pivot) and multiplies the live registers
+  __builtin_amdgcn_sched_barrier(0);
+  const T p = read_lane(a[J], J);  // pivot A[J][J]
   ok = ok && (p > T(0));
   const T s = rsqrt_nr(p);
-  // row J for the lane's column: A[J][lr] lives in lane lr + 16*sg
-  const T ujc = __shfl(a[sv], lr + 16 * sg) * s;
-  const T ejc = __shfl(e[sv], lr + 16 * sg) * s;
-  // A[J][r] for the lane's rows r = 4lk+v: by symmetry of the trailing matrix it is
-  // A[r][J], column J of the same row group -> a row broadcast of lane J
-  T ujr[4];
+  const T ujc = a[J] * s;  // U[J][c]
+  const T ejc = e[J] * s;  // row J of E, scaled
+  a[J] = ujc;
+  e[J] = ejc;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) ujr[v] = row_bcast<J>(a[v]) * s;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = 4 * lk + v;
-    if (r > J) {
-      a[v] -= ujr[v] * ujc;
-      e[v] -= ujr[v] * ejc;
-    } else if (r == J) {
-      a[v] = ujc;
-      e[v] = ejc;
-    }
+  for (int r = J + 1; r < 16; ++r) {
+    // U[J][r] = A[J][r] s = A[r][J] s (the trailing matrix stays symmetric): lane J's a[r]
+    const T ujr = row_bcast<J>(a[r]) * s;
+    a[r] -= ujr * ujc;
+    e[r] -= ujr * ejc;
   }
-  if constexpr (J < 15) diag_step<T, J + 1>(a, e, ok, lr, lk);
+  // E is consumed only by the final store; without this the compiler defers all of
+  // its updates to the end and keeps every multiplier of every step live
+#pragma unroll
+  for (int r = 0; r < 16; ++r) asm volatile("" : "+v"(e[r]));
+  if constexpr (J < 15) diag_step<T, J + 1>(a, e, ok);
 }
 
 // Wave-level Cholesky + inverse of one 16x16 SPD tile held (upper triangle) in LDS.
-// Lane l owns column l%16, rows 4*(l/16)..+3, of the full symmetric tile.  Row
-// operations are applied to [A | I]; at the end the right half is U^{-T}.  Writes
-// Uinv = (U^{-T})^T into the upper triangle of the output tile (strict lower part
+// Row operations are applied to [A | I]; at the end the right half is E = U^{-T}.
+// Writes Uinv = E^T into the upper triangle of the output tile (strict lower part
 // exact zeros).  Returns false if a pivot was not positive.
 template <typename T>
-__device__ inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
-  const int l = lane_id();
-  const int lr = l & 15, lk = l >> 4;
-  T a[4], e[4];
+__device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
+  const int c = lane_id() & 15;
+  T a[16], e[16];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = 4 * lk + v;
-    a[v] = (r <= lr) ? at[lr * lda + r] : at[r * lda + lr];
-    e[v] = (r == lr) ? T(1) : T(0);
+  for (int r = 0; r < 16; ++r) {
+    a[r] = (r <= c) ? at[c * lda + r] : at[r * lda + c];
+    e[r] = (r == c) ? T(1) : T(0);
   }
   bool ok = true;
-  diag_step<T, 0>(a, e, ok, lr, lk);
+  diag_step<T, 0>(a, e, ok);
+  if (lane_id() < 16) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) ut[(4 * lk + v) * ldu + lr] = e[v];
+    for (int r = 0; r < 16; ++r) ut[r * ldu + c] = e[r];  // Uinv[c][r] = E[r][c]
+  }
   return ok;
 }
 
@@ -248,7 +248,9 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
   constexpr int NN = NB * NB;
   constexpr int NS = TT * (TT + 1) / 2;           // upper tiles of a symmetric block
   constexpr int SQ = (NS + NW - 1) / NW;          // S accumulators per wave
-  constexpr int XQ = (2 * TT * TT + NW - 1) / NW; // trsm output tiles per wave
+  static_assert(NW % TT == 0, "waves must be a multiple of the tile rows");
+  constexpr int XG = NW / TT;                      // wave groups splitting the RHS columns
+  constexpr int XQ = 2 * TT / XG;                  // trsm output tiles per wave
   constexpr int ZQ = (TT * TT + NW - 1) / NW;     // panel tiles per wave
   constexpr int PQ = (TT * TT + NS + NW - 1) / NW;
   typedef typename Mfma<T>::acc_t acc_t;
@@ -302,41 +304,51 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
       fail = 0;
     }
 
-    // [C | W] = Uinv^T [B | Z]
+    // [C | W] = Uinv^T [B | Z].  Output tile (r, c) costs r+1 tile products; wave w
+    // (group g = w / TT, residue rho = w % TT) takes columns g*XQ .. g*XQ+XQ-1 with
+    // r = (rho - c) mod TT, so every wave does the same amount of MFMA work.
     const int c0 = hasC ? 0 : TT;
     const int c1 = hasL ? 2 * TT : TT;
-    const int ntile = TT * (c1 - c0);
+    const int xg = w / TT, rho = w % TT;
     acc_t X[XQ];
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
-      const int task = w + NW * q;
+      const int c = xg * XQ + q;
+      const int r = ((rho - c) % TT + TT) % TT;
       X[q] = Mfma<T>::zero();
-      if (task < ntile) {
-        const int r = task % TT, c = c0 + task / TT;
+      if (c >= c0 && c < c1) {
         for (int t = 0; t <= r; ++t)
           tile_mma<T, true, false, false, 16>(X[q], Ubuf + tile_off(t, r, LD), LD,
-                                          Rbuf + tile_off(t, c, LD), LD);
+                                              Rbuf + tile_off(t, c, LD), LD);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
-      const int task = w + NW * q;
-      if (task < ntile) {
-        const int r = task % TT, c = c0 + task / TT;
-        tile_store<T>(Rbuf + tile_off(r, c, LD), LD, X[q]);
-        if (c < TT)
-          tile_store<T>(C + (long long)blk * NN + tile_off(r, c, NB), NB, X[q]);
-        else
-          tile_store<T>(W + (long long)blk * NN + tile_off(r, c - TT, NB), NB, X[q]);
-      }
-    }
-    for (int idx = tid; idx < NN; idx += NT) {
-      const int r = idx % NB, c = idx / NB;
-      if ((r >> 4) <= (c >> 4)) U[(long long)blk * NN + idx] = Ubuf[c * LD + r];
+      const int c = xg * XQ + q;
+      const int r = ((rho - c) % TT + TT) % TT;
+      if (c >= c0 && c < c1) tile_store<T>(Rbuf + tile_off(r, c, LD), LD, X[q]);
     }
     if (!last) sa.template store<T, LD, false>(Abuf);  // A_{blk+1} (prefetched)
     __syncthreads();
+    // factors to HBM as flat 16-byte stores: C (B slot), W (panel slot), Uinv upper tiles
+    {
+      typedef T vt __attribute__((ext_vector_type(16 / sizeof(T))));
+      constexpr int V = 16 / sizeof(T);
+      for (int idx = tid * V; idx < NN; idx += NT * V) {
+        const int row = idx % NB, col = idx / NB;
+        vt u, cc, ww;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          u[e] = Ubuf[col * LD + row + e];
+          cc[e] = Rbuf[col * LD + row + e];
+          ww[e] = Rbuf[(NB + col) * LD + row + e];
+        }
+        if ((row >> 4) <= (col >> 4)) *(vt*)(U + (long long)blk * NN + idx) = u;
+        if (hasC) *(vt*)(C + (long long)blk * NN + idx) = cc;
+        if (hasL) *(vt*)(W + (long long)blk * NN + idx) = ww;
+      }
+    }
 
     // S_L += W^T W (upper tiles, persistent per wave)
     if (hasL) {

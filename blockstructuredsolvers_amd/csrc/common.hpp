@@ -75,15 +75,18 @@ __device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restric
   static_assert(K % 16 == 0, "K must be a multiple of 16");
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
-  T a[K / 4], b[K / 4];
 #pragma unroll
-  for (int q = 0; q < K / 4; ++q) {
-    const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
-    a[q] = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
-    b[q] = TB ? yb[k * ldy + lr] : yb[lr * ldy + k];
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    T a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + 4 * lk + q;
+      a[q] = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
+      b[q] = TB ? yb[k * ldy + lr] : yb[lr * ldy + k];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc = Mfma<T>::mma(NEG ? -a[q] : a[q], b[q], acc);
   }
-#pragma unroll
-  for (int q = 0; q < K / 4; ++q) acc = Mfma<T>::mma(NEG ? -a[q] : a[q], b[q], acc);
 }
 
 // Load / store a D-layout tile from/to a column-major buffer (element (0,0) at p).
