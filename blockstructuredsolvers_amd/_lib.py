@@ -41,6 +41,8 @@ KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3
 _VP, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 SIGNATURES = {
     "bss_plan_create": ([_I, _I, _I, _I, _I, _I, _I, ctypes.POINTER(_VP)], _I),
+    "bss_plan_create_group": ([_I, ctypes.POINTER(_I), _I, _I, _I, _I, _I, ctypes.POINTER(_VP)], _I),
+    "bss_plan_system_sizes": ([_VP, ctypes.POINTER(_I)], _I),
     "bss_set_option": ([_VP, _I, _D], _I),
     "bss_plan_get": ([_VP, _I, ctypes.POINTER(ctypes.c_int64)], _I),
     "bss_plan_bind": ([_VP, _VP, _VP, _VP], _I),

@@ -28,23 +28,14 @@
 #pragma once
 
 #include "common.hpp"
+#include "plan_tables.hpp"
 
 namespace bss {
 
-template <typename T>
 struct FactorLevelArgs {
-  const void* A_in;  // diagonal blocks (input element type), N per system
-  const void* B_in;  // couplings (input element type), N-1 per system
-  T* U_out;          // Uinv, N per system (may alias A_in)
-  T* C_out;          // C, N-1 per system (may alias B_in)
-  T* W_out;          // panel W, N per system
-  T* SR;             // P per system: C_last^T C_last of the chain left of separator p
-  T* SL;             // P per system: sum W^T W of the chain right of separator p
-  T* Bnext;          // P-1 per system: reduced coupling separator p -> p+1
-  const int4* chains;  // {start, len, sepL, sepR}, sep = -1 when absent
-  const int* orig;     // N: level-0 block index of each block (info reporting)
-  int* info;           // per system, atomicMin of 1-based failing block index
-  long long sysA, sysB, sysU, sysC, sysW, sysS, sysBn;  // per-system element strides
+  const ChainRec* chains;  // one workgroup per chain
+  const LevelSys* sys;     // per-system buffers of this level
+  int* info;               // per system, atomicMin of the 1-based failing block index
 };
 
 template <typename T>
@@ -241,7 +232,7 @@ struct BlockStage {
 };
 
 template <typename Tin, typename T, int NB, int NW>
-__global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T> p) {
+__global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs p) {
   constexpr int TT = NB / 16;
   constexpr int LD = NB + LdsPad<T>::value;
   constexpr int NT = 64 * NW;
@@ -260,16 +251,17 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
   __shared__ T Rbuf[2 * NB * LD];  // columns [0,NB): B -> C ; [NB,2NB): Z -> W
   __shared__ int fail;
 
-  const int4 cd = p.chains[blockIdx.x];
-  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  const ChainRec cd = p.chains[blockIdx.x];
+  const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
   if (len <= 0) return;
-  const int sys = blockIdx.y;
+  const LevelSys S = p.sys[cd.slot];
+  const int sys = S.sys;
   const bool hasL = sepL >= 0, hasR = sepR >= 0;
-  const Tin* A = (const Tin*)p.A_in + sys * p.sysA;
-  const Tin* B = (const Tin*)p.B_in + sys * p.sysB;
-  T* U = p.U_out + sys * p.sysU;
-  T* C = p.C_out + sys * p.sysC;
-  T* W = p.W_out + sys * p.sysW;
+  const Tin* A = (const Tin*)S.A;
+  const Tin* B = (const Tin*)S.B;
+  T* U = (T*)S.U;
+  T* C = (T*)S.C;
+  T* W = (T*)S.W;
   const int w = wave_id();
   const int tid = threadIdx.x;
 
@@ -299,7 +291,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
 
     potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
     if (tid == 0 && fail) {
-      const int ob = p.orig ? p.orig[blk] : blk;
+      const int ob = S.orig ? S.orig[blk] : blk;
       atomicMin(p.info + sys, ob + 1);
       fail = 0;
     }
@@ -406,7 +398,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
     } else {
       // reduced-system contributions of the last block
       if (hasR) {
-        T* sr = p.SR + sys * p.sysS + (long long)sepR * NN;
+        T* sr = (T*)S.SR + (long long)sepR * NN;
         for (int task = w; task < NS; task += NW) {
           int s = task, a = 0;
           while (s >= TT - a) {
@@ -419,7 +411,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
           tile_store<T>(sr + tile_off(a, b, NB), NB, acc);
         }
         if (hasL) {
-          T* bn = p.Bnext + sys * p.sysBn + (long long)sepL * NN;
+          T* bn = (T*)S.Bn + (long long)sepL * NN;
           for (int task = w; task < TT * TT; task += NW) {
             const int a = task % TT, b = task / TT;
             acc_t acc = Mfma<T>::zero();
@@ -432,7 +424,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs<T
     }
   }
   if (hasL) {
-    T* sl = p.SL + sys * p.sysS + (long long)sepL * NN;
+    T* sl = (T*)S.SL + (long long)sepL * NN;
 #pragma unroll
     for (int q = 0; q < SQ; ++q) {
       int s = w + NW * q;

@@ -15,21 +15,14 @@
 #pragma once
 
 #include "common.hpp"
+#include "plan_tables.hpp"
 
 namespace bss {
 
-template <typename T>
 struct SolveLevelArgs {
-  const T* U;   // Uinv blocks (upper 16x16 tiles)
-  const T* C;   // couplings C
-  const T* W;   // panels W
-  T* d;         // level vectors, n x nrhs per block, in place y -> x
-  T* rR;        // forward: C_last^T y_last per separator
-  T* rL;        // forward: sum W^T y per separator
-  const T* xsep;  // backward: next level's solution, n x nrhs per separator
-  const int4* chains;
+  const ChainRec* chains;
+  const LevelSys* sys;  // U, C, W, d; forward writes rR/rL, backward reads dn (x_sep)
   int nrhs;
-  long long sysU, sysC, sysW, sysD, sysR, sysX;  // per-system element strides
 };
 
 template <typename T>
@@ -72,7 +65,7 @@ __device__ inline void load_vec(const T* g, T* v, int nrhs) {
 }
 
 template <typename T, int NB>
-__global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs<T> p) {
+__global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p) {
   constexpr int TT = NB / 16;
   constexpr int NT = 64 * TT;
   constexpr int LDV = NB + VecPad<T>::value;
@@ -81,16 +74,16 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs<T>
   __shared__ T Y[16 * LDV];
   __shared__ T R[16 * LDV];
 
-  const int4 cd = p.chains[blockIdx.x];
-  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  const ChainRec cd = p.chains[blockIdx.x];
+  const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
   if (len <= 0) return;
-  const int sys = blockIdx.y;
+  const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
-  const T* U = p.U + sys * p.sysU;
-  const T* C = p.C + sys * p.sysC;
-  const T* W = p.W + sys * p.sysW;
-  T* d = p.d + sys * p.sysD;
+  const T* U = (const T*)S.U;
+  const T* C = (const T*)S.C;
+  const T* W = (const T*)S.W;
+  T* d = (T*)S.d;
   const int w = wave_id();
 
   zero_vec<T, LDV, NT>(Y);
@@ -119,13 +112,13 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs<T>
     const long long last = start + len - 1;
     acc_t acc = Mfma<T>::zero();
     tile_mma<T, true, false, false, NB>(acc, C + last * NN + 16 * w * NB, NB, Y, LDV);
-    store_rhs_tile<T, NB>(p.rR + sys * p.sysR + sepR * ND, w, nrhs, acc);
+    store_rhs_tile<T, NB>((T*)S.rR + sepR * ND, w, nrhs, acc);
   }
-  if (sepL >= 0) store_rhs_tile<T, NB>(p.rL + sys * p.sysR + sepL * ND, w, nrhs, accL);
+  if (sepL >= 0) store_rhs_tile<T, NB>((T*)S.rL + sepL * ND, w, nrhs, accL);
 }
 
 template <typename T, int NB>
-__global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs<T> p) {
+__global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p) {
   constexpr int TT = NB / 16;
   constexpr int NT = 64 * TT;
   constexpr int LDV = NB + VecPad<T>::value;
@@ -135,17 +128,17 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs<T
   __shared__ T XL[16 * LDV];
   __shared__ T R[16 * LDV];
 
-  const int4 cd = p.chains[blockIdx.x];
-  const int start = cd.x, len = cd.y, sepL = cd.z, sepR = cd.w;
+  const ChainRec cd = p.chains[blockIdx.x];
+  const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
   if (len <= 0) return;
-  const int sys = blockIdx.y;
+  const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
-  const T* U = p.U + sys * p.sysU;
-  const T* C = p.C + sys * p.sysC;
-  const T* W = p.W + sys * p.sysW;
-  T* d = p.d + sys * p.sysD;
-  const T* xs = p.xsep ? p.xsep + sys * p.sysX : nullptr;
+  const T* U = (const T*)S.U;
+  const T* C = (const T*)S.C;
+  const T* W = (const T*)S.W;
+  T* d = (T*)S.d;
+  const T* xs = (const T*)S.dn;
   const int w = wave_id();
 
   zero_vec<T, LDV, NT>(Xn);

@@ -2,10 +2,10 @@
 // mixed-precision path below its FP64 top-level input).
 #include "kernels_impl.hpp"
 namespace bss {
-template hipError_t launch_factor<float, float>(int, int, const FactorLevelArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_forward<float>(int, const SolveLevelArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_backward<float>(int, const SolveLevelArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_assemble_diag<float, float>(const float*, const float*, const float*, const int*, float*, int, int, int, long long, long long, long long, hipStream_t);
-template hipError_t launch_assemble_rhs<float>(const float*, const float*, const float*, const int*, float*, int, int, int, long long, long long, long long, hipStream_t);
-template hipError_t launch_scatter_sep<float>(const float*, const int*, float*, int, int, int, long long, long long, hipStream_t);
+template hipError_t launch_factor<float, float>(int, int, const FactorLevelArgs&, int, hipStream_t);
+template hipError_t launch_forward<float>(int, const SolveLevelArgs&, int, hipStream_t);
+template hipError_t launch_backward<float>(int, const SolveLevelArgs&, int, hipStream_t);
+template hipError_t launch_assemble_diag<float, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
+template hipError_t launch_assemble_rhs<float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
+template hipError_t launch_scatter_sep<float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
 }  // namespace bss

@@ -2,6 +2,6 @@
 // factor kernel; reduced diagonals assembled in FP64 from the FP64 separator blocks.
 #include "kernels_impl.hpp"
 namespace bss {
-template hipError_t launch_factor<double, float>(int, int, const FactorLevelArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_assemble_diag<double, float>(const double*, const float*, const float*, const int*, float*, int, int, int, long long, long long, long long, hipStream_t);
+template hipError_t launch_factor<double, float>(int, int, const FactorLevelArgs&, int, hipStream_t);
+template hipError_t launch_assemble_diag<double, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
 }  // namespace bss

@@ -11,52 +11,48 @@
 #pragma once
 
 #include "common.hpp"
+#include "plan_tables.hpp"
 
 namespace bss {
 
+// one workgroup per separator (all systems of the level in one launch)
 template <typename Tin, typename T>
-__global__ void __launch_bounds__(256) assemble_diag_kernel(const Tin* __restrict__ A,
-                                                             const T* __restrict__ SR,
-                                                             const T* __restrict__ SL,
-                                                             const int* __restrict__ seppos,
-                                                             T* __restrict__ Anext, int nn,
-                                                             long long sysA, long long sysS,
-                                                             long long sysN) {
-  const int p = blockIdx.x, sys = blockIdx.y;
-  const Tin* a = A + sys * sysA + (long long)seppos[p] * nn;
-  const T* sr = SR + sys * sysS + (long long)p * nn;
-  const T* sl = SL + sys * sysS + (long long)p * nn;
-  T* an = Anext + sys * sysN + (long long)p * nn;
+__global__ void __launch_bounds__(256) assemble_diag_kernel(const SepRec* __restrict__ seps,
+                                                             const LevelSys* __restrict__ sys,
+                                                             int nn) {
+  const SepRec sr = seps[blockIdx.x];
+  const LevelSys& S = sys[sr.slot];
+  const Tin* a = (const Tin*)S.A + (long long)sr.pos * nn;
+  const T* r = (const T*)S.SR + (long long)sr.p * nn;
+  const T* l = (const T*)S.SL + (long long)sr.p * nn;
+  T* an = (T*)S.An + (long long)sr.p * nn;
   for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-    const double v = (double)a[e] - (double)sr[e] - (double)sl[e];
+    const double v = (double)a[e] - (double)r[e] - (double)l[e];
     an[e] = (T)v;
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) assemble_rhs_kernel(const T* __restrict__ d,
-                                                            const T* __restrict__ rR,
-                                                            const T* __restrict__ rL,
-                                                            const int* __restrict__ seppos,
-                                                            T* __restrict__ dnext, int nd,
-                                                            long long sysD, long long sysR,
-                                                            long long sysN) {
-  const int p = blockIdx.x, sys = blockIdx.y;
-  const T* src = d + sys * sysD + (long long)seppos[p] * nd;
-  const T* a = rR + sys * sysR + (long long)p * nd;
-  const T* b = rL + sys * sysR + (long long)p * nd;
-  T* dst = dnext + sys * sysN + (long long)p * nd;
+__global__ void __launch_bounds__(256) assemble_rhs_kernel(const SepRec* __restrict__ seps,
+                                                            const LevelSys* __restrict__ sys,
+                                                            int nd) {
+  const SepRec sr = seps[blockIdx.x];
+  const LevelSys& S = sys[sr.slot];
+  const T* src = (const T*)S.d + (long long)sr.pos * nd;
+  const T* a = (const T*)S.rR + (long long)sr.p * nd;
+  const T* b = (const T*)S.rL + (long long)sr.p * nd;
+  T* dst = (T*)S.dn + (long long)sr.p * nd;
   for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = src[e] - a[e] - b[e];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) scatter_sep_kernel(const T* __restrict__ x,
-                                                           const int* __restrict__ seppos,
-                                                           T* __restrict__ d, int nd,
-                                                           long long sysX, long long sysD) {
-  const int p = blockIdx.x, sys = blockIdx.y;
-  const T* src = x + sys * sysX + (long long)p * nd;
-  T* dst = d + sys * sysD + (long long)seppos[p] * nd;
+__global__ void __launch_bounds__(256) scatter_sep_kernel(const SepRec* __restrict__ seps,
+                                                           const LevelSys* __restrict__ sys,
+                                                           int nd) {
+  const SepRec sr = seps[blockIdx.x];
+  const LevelSys& S = sys[sr.slot];
+  const T* src = (const T*)S.dn + (long long)sr.p * nd;
+  T* dst = (T*)S.d + (long long)sr.pos * nd;
   for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = src[e];
 }
 

@@ -11,7 +11,6 @@ namespace bss {
 namespace {
 
 thread_local std::string g_mixed_err;
-constexpr int kMaxWG = 1024;  // residual workgroups per system (partial-max slots)
 // per workgroup partials: [0,16) max|r_j|, [16,32) max|x_j|, [32,48) max|d_j|,
 // [48] max row sum of |M| (when requested)
 constexpr int kParts = 64;
@@ -22,7 +21,7 @@ size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 // NaN-propagating max (fmax would drop a NaN residual and report convergence)
 __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
-// Residual pass over the blocks [wg*bpw, (wg+1)*bpw) of one system:
+// Residual pass over one work item (a block range [b0, b1) of one system):
 //   x_new = x_old + dx (FP64; written for the workgroup's own blocks),
 //   r = d - (B_{i-1}^T x_{i-1} + A_i x_i + B_i x_{i+1}) in FP64 on MFMA, stored as FP32
 //   for the next correction solve, plus per-column max |r|, |x|, |d|.
@@ -33,29 +32,30 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // each block's A/B operands are issued before the window update to hide HBM latency.
 template <int NB, bool INIT>
 __global__ void __launch_bounds__(4 * NB)
-    residual_kernel(const double* __restrict__ A, const double* __restrict__ B,
-                    const double* __restrict__ d, const double* __restrict__ xold,
-                    const float* __restrict__ dx, double* __restrict__ xnew,
-                    float* __restrict__ rout, double* __restrict__ part, int N, int nrhs,
-                    int bpw, int nwg, int mnorm) {
+    residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
+                    double* __restrict__ part, int nrhs, int xo_k, int dx_k, int xn_k, int ro_k,
+                    int mnorm) {
   constexpr int TT = NB / 16, NT = 64 * TT, LDV = NB + 2, KQ = NB / 4;
   constexpr long long NN = (long long)NB * NB;
   typedef Mfma<double>::acc_t acc_t;
   __shared__ double X[3][16 * LDV];
   __shared__ double red[4][4 * TT][16];
-  const int sys = blockIdx.y, wg = blockIdx.x, tid = threadIdx.x;
+  const ResItem it = items[blockIdx.x];
+  const ResSys& S = systems[it.sys];
+  const int N = S.N;
+  const int tid = threadIdx.x;
   const int w = wave_id(), l = lane_id(), lr = l & 15, lk = l >> 4;
   const long long ND = (long long)NB * nrhs;
-  const double* As = A + sys * (long long)N * NN;
-  const double* Bs = B + sys * (long long)(N > 0 ? N - 1 : 0) * NN;
-  const double* ds = d + sys * (long long)N * ND;
-  const double* xo = xold ? xold + sys * (long long)N * ND : nullptr;
-  const float* dxs = dx ? dx + sys * (long long)N * ND : nullptr;
-  double* xn = xnew + sys * (long long)N * ND;
-  float* ro = rout + sys * (long long)N * ND;
+  const double* As = S.A;
+  const double* Bs = S.B;
+  const double* ds = S.d;
+  const double* xo = xo_k >= 0 ? S.x[xo_k] : nullptr;
+  const float* dxs = dx_k >= 0 ? S.v[dx_k] : nullptr;
+  double* xn = S.x[xn_k];
+  float* ro = S.v[ro_k];
   for (int e = tid; e < 3 * 16 * LDV; e += NT) (&X[0][0])[e] = 0.0;
   double rmax = 0.0, xmax = 0.0, dmax = 0.0, mmax = 0.0;
-  const int b0 = wg * bpw, b1 = min(N, b0 + bpw);
+  const int b0 = it.b0, b1 = it.b1;
 
   // x_new of block j into its window slot (zero outside the system); own blocks are
   // also written to xnew
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(4 * NB)
       for (int k = 0; k < 4 * TT; ++k)
         for (int cc = 0; cc < 16; ++cc) m = nmax(m, red[3][k][cc]);
     }
-    if (q < 3 || c == 0) part[((long long)sys * nwg + wg) * kParts + (q < 3 ? tid : kMnormSlot)] = m;
+    if (q < 3 || c == 0) part[(long long)blockIdx.x * kParts + (q < 3 ? tid : kMnormSlot)] = m;
   }
 }
 
@@ -155,16 +155,17 @@ constexpr int kRedThreads = 256;
 constexpr int kSlots = 49;  // 3 x 16 column maxima + the |M| row-sum maximum
 
 __global__ void __launch_bounds__(kRedThreads)
-    berr_kernel(const double* __restrict__ part, double* __restrict__ mnorm,
-                double* __restrict__ dnorm, double* __restrict__ berr, int nwg, int nrhs,
-                int init, int want_mnorm) {
+    berr_kernel(const ResSys* __restrict__ systems, const double* __restrict__ part,
+                double* __restrict__ mnorm, double* __restrict__ dnorm,
+                double* __restrict__ berr, int nrhs, int init, int want_mnorm) {
   __shared__ double red[kRedThreads / 64][kSlots];
   const int sys = blockIdx.x, tid = threadIdx.x;
+  const int g0 = systems[sys].item0, g1 = systems[sys].item1;
   double m[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) m[k] = 0.0;
-  for (int g = tid; g < nwg; g += kRedThreads) {
-    const double* pp = part + ((long long)sys * nwg + g) * kParts;
+  for (int g = g0 + tid; g < g1; g += kRedThreads) {
+    const double* pp = part + (long long)g * kParts;
 #pragma unroll
     for (int k = 0; k < 48; ++k) m[k] = nmax(m[k], pp[k]);
     m[48] = nmax(m[48], pp[kMnormSlot]);
@@ -195,76 +196,129 @@ __global__ void __launch_bounds__(kRedThreads)
   berr[sys * nrhs + c] = den > 0.0 ? r / den : r;
 }
 
-void geometry(int N, int* nwg, int* bpw) {
-  *nwg = std::max(1, std::min(kMaxWG, N));
-  *bpw = (N + *nwg - 1) / *nwg;
-  *nwg = (N + *bpw - 1) / *bpw;
-}
+// blocks per residual workgroup: at least 8, and at most 1024 workgroups per system
+int blocks_per_item(int N) { return std::max(8, (N + 1023) / 1024); }
 
 template <int NB>
-hipError_t launch_residual(const MixedState& m, const double* A, const double* B,
-                           const double* d, const double* xold, const float* dx, double* xnew,
-                           float* rout, bool init, int want_mnorm, hipStream_t s) {
+hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, int xn_k,
+                           int ro_k, int want_mnorm, hipStream_t s) {
   if (init)
-    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nwg, m.nsys), dim3(4 * NB), 0, s, A,
-                       B, d, xold, dx, xnew, rout, m.part, m.N, m.nrhs, m.bpw, m.nwg, 0);
+    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nitems), dim3(4 * NB), 0, s, m.items,
+                       m.sys_dev, m.part, m.nrhs, -1, -1, xn_k, ro_k, 0);
   else
-    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nwg, m.nsys), dim3(4 * NB), 0, s, A,
-                       B, d, xold, dx, xnew, rout, m.part, m.N, m.nrhs, m.bpw, m.nwg, want_mnorm);
+    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
+                       m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   return hipGetLastError();
 }
 
-hipError_t residual(const MixedState& m, const double* A, const double* B, const double* d,
-                    const double* xold, const float* dx, double* xnew, float* rout, int init,
+hipError_t residual(const MixedState& m, bool init, int xo_k, int dx_k, int xn_k, int ro_k,
                     int want_mnorm, hipStream_t s) {
   hipError_t e;
   switch (m.n) {
-    case 16: e = launch_residual<16>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
-    case 32: e = launch_residual<32>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
-    case 48: e = launch_residual<48>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
-    case 64: e = launch_residual<64>(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s); break;
+    case 16: e = launch_residual<16>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+    case 32: e = launch_residual<32>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+    case 48: e = launch_residual<48>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+    case 64: e = launch_residual<64>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(kRedThreads), 0, s, m.part, m.mnorm, m.dnorm,
-                     m.berr, m.nwg, m.nrhs, init, want_mnorm);
+  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(kRedThreads), 0, s, m.sys_dev, m.part,
+                     m.mnorm, m.dnorm, m.berr, m.nrhs, init ? 1 : 0, want_mnorm);
   return hipGetLastError();
+}
+
+constexpr int kKindResidual = 6;  // BSS_K_RESIDUAL in bss.h
+
+hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, bool init, int xo_k,
+                         int dx_k, int xn_k, int ro_k, int want_mnorm, hipStream_t s) {
+  void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindResidual, s) : nullptr;
+  hipError_t e = residual(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s);
+  if (scope) h.prof_end(scope);
+  return e;
 }
 
 }  // namespace
 
-size_t MixedState::bytes(int N, int n, int nrhs, int nsys) {
-  int nwg, bpw;
-  geometry(N, &nwg, &bpw);
-  const size_t nd = (size_t)N * n * nrhs * nsys;
-  return 2 * al(nd * 8) + al(nd * 4) + al((size_t)nsys * nwg * kParts * 8) + al(nsys * 8) +
-         al((size_t)nsys * 16 * 8) + al((size_t)nsys * 16 * 8);
+size_t MixedState::bytes(const std::vector<int>& Ns, int n, int nrhs) {
+  size_t b = 0, items = 0;
+  const size_t S = Ns.size();
+  for (int N : Ns) {
+    const size_t nd = (size_t)N * n * nrhs;
+    b += 2 * al(nd * 8) + 2 * al(nd * 4);
+    items += (N + blocks_per_item(N) - 1) / blocks_per_item(N);
+  }
+  b += al(items * sizeof(ResItem)) + al(S * sizeof(ResSys)) + al(items * kParts * 8);
+  b += al(S * 8) + al(S * 16 * 8) + al(S * 16 * 8);
+  return b;
 }
 
-void MixedState::attach(char* base, int N_, int n_, int nrhs_, int nsys_) {
-  N = N_;
+int MixedState::attach(char* base, const std::vector<int>& Ns_, int n_, int nrhs_) {
+  Ns = Ns_;
   n = n_;
   nrhs = nrhs_;
-  nsys = nsys_;
-  geometry(N, &nwg, &bpw);
-  const size_t nd = (size_t)N * n * nrhs * nsys;
+  nsys = (int)Ns.size();
   size_t o = 0;
-  x[0] = (double*)(base + o);
-  o += al(nd * 8);
-  x[1] = (double*)(base + o);
-  o += al(nd * 8);
-  corr = (float*)(base + o);
-  o += al(nd * 4);
-  part = (double*)(base + o);
-  o += al((size_t)nsys * nwg * kParts * 8);
-  mnorm = (double*)(base + o);
-  o += al(nsys * 8);
-  dnorm = (double*)(base + o);
-  o += al((size_t)nsys * 16 * 8);
-  berr = (double*)(base + o);
-  if (!host_berr && hipHostMalloc((void**)&host_berr, (size_t)nsys * 16 * sizeof(double)) != hipSuccess)
+  auto take = [&](size_t bytes) {
+    char* p = base + o;
+    o += al(bytes);
+    return p;
+  };
+  sys_host.assign(nsys, ResSys{});
+  std::vector<ResItem> it;
+  for (int s = 0; s < nsys; ++s) {
+    const int N = Ns[s];
+    const size_t nd = (size_t)N * n * nrhs;
+    ResSys& r = sys_host[s];
+    r.x[0] = (double*)take(nd * 8);
+    r.x[1] = (double*)take(nd * 8);
+    r.v[0] = (float*)take(nd * 4);
+    r.v[1] = (float*)take(nd * 4);
+    r.N = N;
+    r.item0 = (int)it.size();
+    const int bpw = blocks_per_item(N);
+    for (int b0 = 0; b0 < N; b0 += bpw) it.push_back({s, b0, std::min(N, b0 + bpw), 0});
+    r.item1 = (int)it.size();
+  }
+  nitems = (int)it.size();
+  items = (ResItem*)take(it.size() * sizeof(ResItem));
+  sys_dev = (ResSys*)take(nsys * sizeof(ResSys));
+  part = (double*)take((size_t)nitems * kParts * 8);
+  mnorm = (double*)take(nsys * 8);
+  dnorm = (double*)take((size_t)nsys * 16 * 8);
+  berr = (double*)take((size_t)nsys * 16 * 8);
+  if (hipMemcpy(items, it.data(), it.size() * sizeof(ResItem), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    g_mixed_err = "work-item upload failed";
+    return -1;
+  }
+  if (!host_berr &&
+      hipHostMalloc((void**)&host_berr, (size_t)nsys * 16 * sizeof(double)) != hipSuccess) {
     host_berr = nullptr;
+    g_mixed_err = "pinned host allocation failed";
+    return -1;
+  }
   last_berr.assign((size_t)nsys * nrhs, 0.0);
+  return 0;
+}
+
+int MixedState::bind(const double* A, const double* B, const double* d) {
+  size_t oa = 0, ob = 0, od = 0;
+  const size_t nn = (size_t)n * n, ndv = (size_t)n * nrhs;
+  for (int s = 0; s < nsys; ++s) {
+    const int N = Ns[s];
+    sys_host[s].A = A + oa;
+    sys_host[s].B = B ? B + ob : nullptr;
+    sys_host[s].d = d + od;
+    oa += (size_t)N * nn;
+    ob += (size_t)std::max(N - 1, 0) * nn;
+    od += (size_t)N * ndv;
+  }
+  if (hipMemcpy(sys_dev, sys_host.data(), nsys * sizeof(ResSys), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    g_mixed_err = "system-table upload failed";
+    return -1;
+  }
+  return 0;
 }
 
 void MixedState::release() {
@@ -272,53 +326,27 @@ void MixedState::release() {
   host_berr = nullptr;
 }
 
-int mixed_after_factor(MixedState& m, const double* A, const double* B, int n, int nsys,
-                       hipStream_t s) {
-  (void)A;
-  (void)B;
-  (void)n;
-  (void)nsys;
-  (void)s;
-  m.mnorm_stale = true;  // recomputed by the next solve's first residual pass
-  return 0;
-}
+void mixed_after_factor(MixedState& m) { m.mnorm_stale = true; }
 
-namespace {
-constexpr int kKindResidual = 6;  // BSS_K_RESIDUAL in bss.h
-
-hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, const double* A,
-                         const double* B, const double* d, const double* xold,
-                         const float* dx, double* xnew, float* rout, int init, int want_mnorm,
-                         hipStream_t s) {
-  void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindResidual, s) : nullptr;
-  hipError_t e = residual(m, A, B, d, xold, dx, xnew, rout, init, want_mnorm, s);
-  if (scope) h.prof_end(scope);
-  return e;
-}
-}  // namespace
-
-int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const double* B,
-                double* d, float* v0, int n, int nrhs, int nsys, double tol, int max_iters,
+int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
                 int* iters, hipStream_t s) {
-  (void)n;
-  (void)nrhs;
-  float* v[2] = {v0, m.corr};
   int cur = 0, xa = 0, it = 0;
-  hipError_t e = residual_prof(m, h, A, B, d, nullptr, nullptr, m.x[0], v[0], 1, 0, s);
+  // first pass: x = 0, r = d (as FP32 into v[0]), ||d||
+  hipError_t e = residual_prof(m, h, true, -1, -1, 0, 0, 0, s);
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;
   }
   while (true) {
-    if (h.solve32(h.ctx, v[cur], s) != 0) {
+    if (h.solve32(h.ctx, cur, s) != 0) {
       g_mixed_err = "FP32 solve failed";
       return -1;
     }
     const int want_mnorm = m.mnorm_stale ? 1 : 0;
-    e = residual_prof(m, h, A, B, d, m.x[xa], v[cur], m.x[1 - xa], v[1 - cur], 0, want_mnorm, s);
+    e = residual_prof(m, h, false, xa, cur, 1 - xa, 1 - cur, want_mnorm, s);
     m.mnorm_stale = false;
     if (e == hipSuccess)
-      e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * nsys * m.nrhs,
+      e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * m.nsys * m.nrhs,
                          hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -330,7 +358,7 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
     ++it;
     double worst = 0.0;
     bool nan = false;
-    for (int k = 0; k < nsys * m.nrhs; ++k) {
+    for (int k = 0; k < m.nsys * m.nrhs; ++k) {
       const double b = m.host_berr[k];
       m.last_berr[k] = b;
       if (b != b) nan = true;
@@ -338,11 +366,16 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const 
     }
     if ((!nan && worst <= tol) || it >= max_iters) break;
   }
-  e = hipMemcpyAsync(d, m.x[xa], sizeof(double) * (size_t)m.N * m.n * m.nrhs * nsys,
-                     hipMemcpyDeviceToDevice, s);
-  if (e != hipSuccess) {
-    g_mixed_err = hipGetErrorString(e);
-    return -1;
+  // the final iterate into the caller's d (packed like d)
+  size_t od = 0;
+  for (int sy = 0; sy < m.nsys; ++sy) {
+    const size_t nd = (size_t)m.Ns[sy] * m.n * m.nrhs;
+    e = hipMemcpyAsync(d + od, m.sys_host[sy].x[xa], nd * 8, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      g_mixed_err = hipGetErrorString(e);
+      return -1;
+    }
+    od += nd;
   }
   *iters = it;
   return 0;

@@ -15,27 +15,51 @@
 
 namespace bss {
 
+// one system's level-0 data for the residual pass
+struct ResSys {
+  const double* A;
+  const double* B;
+  const double* d;
+  double* x[2];  // FP64 iterates (ping-pong)
+  float* v[2];   // FP32 right-hand side / correction vectors (ping-pong)
+  int N;
+  int item0, item1;  // residual work items of this system
+  int pad;
+};
+
+struct ResItem {
+  int sys, b0, b1, pad;
+};
+
 struct MixedState {
-  int N = 0, n = 0, nrhs = 0, nsys = 0, nwg = 0, bpw = 0;
-  double* x[2] = {nullptr, nullptr};  // FP64 iterates (ping-pong), N*n*nrhs per system
-  float* corr = nullptr;              // second FP32 vector (ping-pong with the level-0 d)
-  double* part = nullptr;             // per-workgroup partial maxima [sys][wg][3][16]
-  double* mnorm = nullptr;            // ||M||_inf per system
-  double* dnorm = nullptr;            // ||d_j||_inf per system and column
-  double* berr = nullptr;             // backward error per system and column
-  double* host_berr = nullptr;        // pinned mirror of berr
-  bool mnorm_stale = true;            // ||M|| must be recomputed (new factorisation)
+  int n = 0, nrhs = 0, nsys = 0, nitems = 0;
+  std::vector<int> Ns;
+  std::vector<ResSys> sys_host;
+  ResSys* sys_dev = nullptr;
+  ResItem* items = nullptr;
+  double* part = nullptr;   // per work item partial maxima
+  double* mnorm = nullptr;  // ||M||_inf per system
+  double* dnorm = nullptr;  // ||d_j||_inf per system and column
+  double* berr = nullptr;   // backward error per system and column
+  double* host_berr = nullptr;  // pinned mirror of berr
+  bool mnorm_stale = true;      // ||M|| must be recomputed (new factorisation)
   std::vector<double> last_berr;
 
-  static size_t bytes(int N, int n, int nrhs, int nsys);
-  void attach(char* base, int N, int n, int nrhs, int nsys);
+  // device bytes for systems of sizes Ns (N_use per system)
+  static size_t bytes(const std::vector<int>& Ns, int n, int nrhs);
+  // carve the workspace at `base`; builds and uploads the work-item table (0 = ok)
+  int attach(char* base, const std::vector<int>& Ns, int n, int nrhs);
+  // (re)bind the caller's FP64 A, B, d (packed systems); uploads the system table
+  int bind(const double* A, const double* B, const double* d);
+  float* v_of(int sys, int k) const { return sys_host[sys].v[k]; }
   void release();
   ~MixedState() { release(); }
 };
 
 struct MixedSolveHooks {
   void* ctx = nullptr;
-  int (*solve32)(void* ctx, float* d0, hipStream_t s) = nullptr;
+  // FP32 recursive solve in place in the level-0 vectors of ping-pong set k
+  int (*solve32)(void* ctx, int k, hipStream_t s) = nullptr;
   // optional profiler brackets around the residual launches (kind = BSS_K_RESIDUAL)
   void* (*prof_begin)(void* ctx, int kind, hipStream_t s) = nullptr;
   void (*prof_end)(void* scope) = nullptr;
@@ -43,14 +67,12 @@ struct MixedSolveHooks {
 
 // Marks ||M||_inf stale after a factorisation; the next solve's first residual pass
 // recomputes it from the operands it reads anyway.
-int mixed_after_factor(MixedState& m, const double* A, const double* B, int n, int nsys,
-                       hipStream_t s);
+void mixed_after_factor(MixedState& m);
 
 // Refinement loop: x_0 = 0; repeat { dx = M32^{-1} r ; x += dx ; r = d - M x } until
 // max berr <= tol or max_iters solves.  On exit d holds x.  Synchronises the stream
 // once per iteration to test convergence.
-int mixed_solve(MixedState& m, const MixedSolveHooks& h, const double* A, const double* B,
-                double* d, float* v0, int n, int nrhs, int nsys, double tol, int max_iters,
+int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
                 int* iters, hipStream_t s);
 
 const char* mixed_last_error();

@@ -45,6 +45,25 @@ def _stream_handle(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
+                 tol, max_iters) -> None:
+    if chain_len is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
+    if base_max is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_BASE_MAX, float(base_max))
+    if chain_len_levels is not None:
+        for k, m in enumerate(list(chain_len_levels)[:16]):
+            L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN_LEVEL + k, float(m))
+    if factor_waves is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_FACTOR_WAVES, float(factor_waves))
+    if preserve_input:
+        L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
+    if tol is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_TOL, float(tol))
+    if max_iters is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_MAX_ITERS, float(max_iters))
+
+
 class BlockTriDiagData_batched:
     """Counterpart of ``BlockTriDiagData_batched`` (src/gpu_batched.jl:1-71): the
     user-visible tensors plus an opaque libbss plan holding the schedule and the
@@ -71,21 +90,8 @@ class BlockTriDiagData_batched:
             L.call("bss_plan_create", N, n, nrhs, nsys, code, mode_code, sched_code,
                    ctypes.byref(plan))
         self._plan = plan
-        if chain_len is not None:
-            L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
-        if base_max is not None:
-            L.call("bss_set_option", plan, L.BSS_OPT_BASE_MAX, float(base_max))
-        if chain_len_levels is not None:
-            for k, m in enumerate(list(chain_len_levels)[:16]):
-                L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN_LEVEL + k, float(m))
-        if factor_waves is not None:
-            L.call("bss_set_option", plan, L.BSS_OPT_FACTOR_WAVES, float(factor_waves))
-        if preserve_input:
-            L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
-        if tol is not None:
-            L.call("bss_set_option", plan, L.BSS_OPT_TOL, float(tol))
-        if max_iters is not None:
-            L.call("bss_set_option", plan, L.BSS_OPT_MAX_ITERS, float(max_iters))
+        _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
+                     tol, max_iters)
         self.N = self._get(L.BSS_Q_N_USE)
         self.dtype = _torch_dtype(code)
         NU = self.N
@@ -169,6 +175,69 @@ class BlockTriDiagData_batched:
             pass
 
 
+class BlockTriDiagData_group(BlockTriDiagData_batched):
+    """An uneven batch: ``nsys`` independent systems of one block size n whose block
+    counts differ (system s has ``Ns[s]`` blocks).  The reference's batch is uniform
+    (src/gpu_batched.jl:165-182); here every system keeps its own level schedule (the
+    one it gets alone) and each level runs all systems that have it in one launch.
+
+    Per-system views replace the single tensors: ``A_tensors[s]`` is (n, n, N_use[s]),
+    ``B_tensors[s]`` (n, n, N_use[s]-1), ``d_tensors[s]`` (n, nrhs, N_use[s]), all
+    column-major blocks packed one system after the other in three device buffers."""
+
+    def __init__(self, Ns: List[int], n: int, T, device, nrhs: int, mode: str,
+                 schedule: str, chain_len: Optional[int], base_max: Optional[int],
+                 preserve_input: bool = False, tol: Optional[float] = None,
+                 max_iters: Optional[int] = None,
+                 chain_len_levels: Optional[List[int]] = None,
+                 factor_waves: Optional[int] = None):
+        Ns = [int(v) for v in Ns]
+        self.Ns_user = Ns
+        self.n = n
+        self.nrhs = nrhs
+        self.nsys = len(Ns)
+        self.mode = mode
+        self.schedule = schedule
+        self.device = torch.device(device)
+        code = _dtype_code(T)
+        mode_code = {"plain": L.BSS_MODE_PLAIN, "mixed": L.BSS_MODE_MIXED}[mode]
+        sched_code = {"auto": L.BSS_SCHED_AUTO, "reference": L.BSS_SCHED_REFERENCE}[schedule]
+        plan = ctypes.c_void_p()
+        arr = (ctypes.c_int * len(Ns))(*Ns)
+        with torch.cuda.device(self.device):
+            L.call("bss_plan_create_group", len(Ns), arr, n, nrhs, code, mode_code,
+                   sched_code, ctypes.byref(plan))
+        self._plan = plan
+        _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
+                     tol, max_iters)
+        nu = (ctypes.c_int * len(Ns))()
+        L.call("bss_plan_system_sizes", plan, nu)
+        self.N_use = [int(v) for v in nu]
+        self.dtype = _torch_dtype(code)
+        totA = sum(self.N_use)
+        totB = sum(max(v - 1, 0) for v in self.N_use)
+        a = torch.zeros((totA, n, n), dtype=self.dtype, device=self.device)
+        b = torch.zeros((max(totB, 1), n, n), dtype=self.dtype, device=self.device)
+        dd = torch.zeros((totA, nrhs, n), dtype=self.dtype, device=self.device)
+        self._a, self._b, self._d = a, b, dd
+        self.A_tensors, self.B_tensors, self.d_tensors = [], [], []
+        oa = ob = 0
+        for N, NU in zip(Ns, self.N_use):
+            nb = max(NU - 1, 0)
+            self.A_tensors.append(a[oa:oa + NU].permute(2, 1, 0))
+            self.B_tensors.append(b[ob:ob + nb].permute(2, 1, 0))
+            self.d_tensors.append(dd[oa:oa + NU].permute(2, 1, 0))
+            if NU > N:  # identity padding (gpu_batched.jl:177-179)
+                a[oa + N - 1:oa + NU] = torch.eye(n, dtype=self.dtype, device=self.device)
+            oa += NU
+            ob += nb
+        with torch.cuda.device(self.device):
+            L.call("bss_plan_bind", plan, a.data_ptr(), b.data_ptr(), dd.data_ptr())
+        self.levels = self._get(L.BSS_Q_LEVELS)
+        self.nchains = [self._get(L.BSS_Q_LEVEL_CHAINS + k) for k in range(self.levels)]
+        self.N = self.N_use[0]
+
+
 def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int = 1,
                        nsys: int = 1, mode: str = "plain", schedule: str = "auto",
                        chain_len: Optional[int] = None,
@@ -206,6 +275,38 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
                                     base_max, preserve_input, tol, max_iters,
                                     chain_len_levels, factor_waves)
+
+
+def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *,
+                             nrhs: int = 1, mode: str = "plain", schedule: str = "auto",
+                             chain_len: Optional[int] = None,
+                             base_max: Optional[int] = None, preserve_input: bool = False,
+                             tol: Optional[float] = None, max_iters: Optional[int] = None,
+                             chain_len_levels: Optional[List[int]] = None,
+                             factor_waves: Optional[int] = None,
+                             tuned: bool = True) -> BlockTriDiagData_group:
+    """Uneven batch: like ``initialize_batched`` but system s has ``Ns[s]`` blocks.
+    Schedule options apply to every system.  With ``tuned=True`` the autotuner cache is
+    used only when it names one configuration for all the sizes in ``Ns``, so each
+    system keeps the schedule it would get alone."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("initialize_batched_group needs a HIP device; none is visible")
+    if len(Ns) < 1:
+        raise ValueError("Ns must name at least one system")
+    device = torch.device(M) if not isinstance(M, torch.device) else M
+    if device.type != "cuda":
+        raise ValueError(f"M must name a GPU device, got {M!r}")
+    if (tuned and schedule == "auto" and chain_len is None and base_max is None
+            and chain_len_levels is None and factor_waves is None):
+        from .autotune import lookup
+        cfgs = [lookup(N, n, T, mode) for N in Ns]
+        if cfgs[0] and all(c == cfgs[0] for c in cfgs):
+            chain_len_levels = cfgs[0].get("chain_len_levels")
+            base_max = cfgs[0].get("base_max")
+            factor_waves = cfgs[0].get("factor_waves")
+    return BlockTriDiagData_group(list(Ns), n, T, device, nrhs, mode, schedule, chain_len,
+                                  base_max, preserve_input, tol, max_iters, chain_len_levels,
+                                  factor_waves)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:

@@ -57,17 +57,19 @@ enum bss_status {
 
 /* Plan queries (bss_plan_get). */
 enum bss_query {
-  BSS_Q_N_USE = 0,          /* blocks per system the caller's A/B/d must hold */
+  BSS_Q_N_USE = 0,          /* blocks per system the caller's A/B/d must hold (system 0
+                               of a group plan) */
   BSS_Q_LEVELS = 1,         /* recursion levels incl. the single-chain base */
   BSS_Q_WORKSPACE_BYTES = 2,
-  BSS_Q_LEVEL_N = 16,       /* + level: blocks at that level */
-  BSS_Q_LEVEL_CHAINS = 48,  /* + level: chains (independent partitions) */
+  BSS_Q_LEVEL_N = 16,       /* + level: blocks at that level (system 0) */
+  BSS_Q_LEVEL_CHAINS = 48,  /* + level: chains of all systems (one workgroup each) */
   BSS_Q_LEVEL_MAXLEN = 80,  /* + level: longest chain */
   BSS_Q_LAST_ITERS = 112,   /* refinement iterations used by the last mixed solve */
   BSS_Q_REFACTORED = 113    /* separator levels re-factored in FP64 (mixed mode) */
 };
 
-/* Options (bss_set_option, before bss_plan_finalize / first factorize). */
+/* Options (bss_set_option).  Schedule and storage options (CHAIN_LEN*, BASE_MAX,
+ * PRESERVE_INPUT) must be set before the first query, bind or factorize. */
 enum bss_option {
   BSS_OPT_TOL = 0,          /* mixed: target normwise backward error (default 1e-12) */
   BSS_OPT_MAX_ITERS = 1,    /* mixed: max refinement iterations (default 10) */
@@ -104,6 +106,24 @@ enum bss_kernel_kind {
  * Supported n: 16, 32, 48, 64 (FP32 and FP64), 1 <= nrhs <= 16. */
 int bss_plan_create(int N, int n, int nrhs, int nsys, int dtype, int mode, int schedule,
                     bss_plan_t* out);
+
+/* Create a plan for `nsys` independent systems of different lengths: system s has
+ * Ns[s] blocks of size n (an uneven batch).  Every system gets its own level
+ * schedule, the same one it would get alone (so its arithmetic, and its result bit
+ * for bit, do not depend on its co-residents), and each recursion level runs all
+ * systems that have it in one launch.  The caller's A, B and d hold the systems
+ * packed one after the other: A = sum_s N_use[s] blocks, B = sum_s (N_use[s]-1)
+ * blocks, d = sum_s N_use[s] vector blocks (N_use from bss_plan_system_sizes).
+ * Generalises initialize_batched (src/gpu_batched.jl:165-182), whose batch is
+ * uniform (one N for all systems). */
+int bss_plan_create_group(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode,
+                          int schedule, bss_plan_t* out);
+
+/* Blocks each system occupies in the packed buffers (N_use[s]; equals Ns[s] except
+ * under BSS_SCHED_REFERENCE, which pads).  Writes nsys values when n_use is not NULL
+ * and returns nsys (or a negative status).  No counterpart in the reference (its
+ * batch is uniform). */
+int bss_plan_system_sizes(bss_plan_t plan, int* n_use);
 
 /* Set a numeric option (see enum bss_option). */
 int bss_set_option(bss_plan_t plan, int option, double value);

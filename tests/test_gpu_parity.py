@@ -191,3 +191,50 @@ def test_batch_equals_alone(mode):
             assert np.abs(xs[k] - xa).max() <= 1e-12 * np.abs(xa).max()
         assert oracle.backward_error(sysm.A, sysm.B, xs[k], sysm.d).max() <= (
             1e-13 if mode == "plain" else 1e-12)
+
+
+def run_gpu_group(systems, T=torch.float64, **kw):
+    from blockstructuredsolvers_amd import initialize_batched_group
+    n, nrhs = systems[0].n, systems[0].nrhs
+    data = initialize_batched_group([s.N for s in systems], n, T, "cuda", nrhs=nrhs, **kw)
+    for s, sysm in enumerate(systems):
+        N = sysm.N
+        data.A_tensors[s][:, :, :N].copy_(torch.from_numpy(sysm.A))
+        data.B_tensors[s][:, :, :N - 1].copy_(torch.from_numpy(sysm.B))
+        data.d_tensors[s][:, :, :N].copy_(torch.from_numpy(sysm.d))
+    factorize(data)
+    solve(data)
+    torch.cuda.synchronize()
+    return [data.d_tensors[s][:, :, :sysm.N].double().cpu().numpy()
+            for s, sysm in enumerate(systems)], data
+
+
+UNEVEN_N = [1, 2, 7, 100, 257, 1000]
+
+
+@pytest.mark.parametrize("schedule", ["auto", "reference"])
+def test_group_uneven_f64(schedule):
+    """An uneven batch (different N per system, one plan) matches the oracle per system
+    and each system's answer is bitwise the one it gets alone."""
+    systems = [workloads.spd_system(N, 32, nrhs=2, seed=300 + N) for N in UNEVEN_N]
+    xs, data = run_gpu_group(systems, schedule=schedule, tuned=False)
+    assert np.all(data.info() == 0)
+    for sysm, x in zip(systems, xs):
+        check_f64(sysm, x)
+        xa, _ = run_gpu(sysm, schedule=schedule, tuned=False)
+        assert np.array_equal(x, xa), sysm.N
+
+
+def test_group_uneven_mixed_and_info():
+    systems = [workloads.spd_system(N, 64, nrhs=1, seed=400 + N) for N in (3, 500, 1200)]
+    xs, data = run_gpu_group(systems, mode="mixed", tuned=False)
+    for sysm, x in zip(systems, xs):
+        assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
+    # a failing block in the middle system is reported for that system only
+    bad = systems[1]
+    A = bad.A.copy(order="F")
+    A[:, :, 37] = -np.eye(64)
+    systems[1] = workloads.BlockSystem(A, bad.B, bad.d)
+    _, data = run_gpu_group(systems, tuned=False)
+    info = data.info()
+    assert info[0] == 0 and info[2] == 0 and info[1] > 0
