@@ -153,7 +153,7 @@ def roofline_for(res, args, nsys):
     avg_ms = total / count
     if name == "factor":
         s_in = 8 if level == 0 else sf
-        # algorithmic: read A, B; write U, C (SURVEY 4 N n^2 s, with input/factor sizes)
+        # algorithmic: read A, B; write U, C (4 N n^2 s, with input/factor sizes)
         bytes_per = 2 * n * n * s_in + 2 * n * n * sf
         flops_per = 7.0 / 3.0 * n ** 3
         alg_bytes = blocks * bytes_per

@@ -32,10 +32,12 @@ BSS_OPT_CHAIN_LEN = 3
 BSS_OPT_BASE_MAX = 4
 BSS_OPT_PRESERVE_INPUT = 5
 BSS_OPT_FACTOR_WAVES = 6
+BSS_OPT_GRAPH = 7
 BSS_OPT_CHAIN_LEN_LEVEL = 32
 
 KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,
-                "backward": 4, "scatter": 5, "residual": 6, "mnorm": 7}
+                "backward": 4, "scatter": 5, "residual": 6, "residual_init": 7,
+                "berr": 8}
 
 # every symbol include/bss.h declares, with its ctypes signature
 _VP, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double

@@ -46,7 +46,15 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 
-constexpr int kInfoNone = 0x7f7f7f7f;  // memset byte 0x7f: "no failure" sentinel
+constexpr int kInfoNone = 0x7f7f7f7f;  // "no failure" sentinel
+
+// Resets per-system status words.  A kernel rather than hipMemsetAsync, so the
+// factorisation can be captured into a hipGraph.
+__global__ void fill_kernel(int* __restrict__ dst, int n, int value) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = value;
+}
+constexpr int kPivNone = 0x7f800000;  // float +inf bits: no pivot seen
 constexpr size_t kNone = ~size_t(0);   // buffer not allocated
 
 // one system at one level: its chain layout and where its buffers live
@@ -55,6 +63,10 @@ struct SysLevel {
   int P = 0;
   size_t oA = kNone, oB = kNone, od = kNone, oW = kNone, oSR = kNone, oSL = kNone,
          orR = kNone, orL = kNone, oU = kNone, oC = kNone, oOrig = kNone;
+  // mixed mode, levels >= 1: FP64 copies used when the separator levels are
+  // re-factored in FP64
+  size_t wA = kNone, wB = kNone, wd = kNone, wW = kNone, wSR = kNone, wSL = kNone,
+         wrR = kNone, wrL = kNone;
 };
 
 struct SysPlan {
@@ -70,6 +82,9 @@ struct GLevel {
   bss::ChainRec* dchains = nullptr;
   bss::SepRec* dseps = nullptr;
   bss::LevelSys* dtab[2] = {nullptr, nullptr};  // [1] differs at level 0 only (d vector)
+  // mixed mode: level >= 1 -> FP64 tables; level 0 -> hand-off tables whose next-level
+  // pointers (An, dn) name the FP64 copies of level 1
+  bss::LevelSys* dtab64[2] = {nullptr, nullptr};
   int maxlen = 0;
 };
 
@@ -111,11 +126,37 @@ struct bss_plan {
   size_t pool_next = 0;
   // table offsets
   struct TabOff {
-    size_t chains = kNone, seps = kNone, tab[2] = {kNone, kNone};
+    size_t chains = kNone, seps = kNone, tab[2] = {kNone, kNone}, tab64[2] = {kNone, kNone};
   };
   std::vector<TabOff> toff;
   size_t info_off = kNone;
+  // separator re-factorisation (mixed mode)
+  size_t piv_off = kNone, widen_off = kNone, narrow_off = kNone;
+  unsigned* dpiv = nullptr;
+  bss::ConvRec<float, double>* dwiden = nullptr;
+  bss::ConvRec<double, float>* dnarrow = nullptr;
+  int nconv = 0;
+  bool pivots_pending = false;  // factorized, pivots not yet inspected
+  unsigned* host_piv = nullptr;  // pinned: [0, nsys) pivot ratios, [nsys, 2 nsys) FP32
+                                 // separator info words, copied after each factorize
+  hipEvent_t piv_event = nullptr;
+  bool sep64 = false;           // separator levels currently factored in FP64
+  hipGraphExec_t g_solve64[2] = {nullptr, nullptr};
+  // hipGraph replay of the launch sequences (option BSS_OPT_GRAPH)
+  bool use_graph = false;
+  hipStream_t cap_stream = nullptr;  // capture happens here, replay on the caller's stream
+  hipGraphExec_t g_factor = nullptr, g_solve[2] = {nullptr, nullptr};
+  void drop_graphs() {
+    for (hipGraphExec_t* g : {&g_factor, &g_solve[0], &g_solve[1], &g_solve64[0], &g_solve64[1]}) {
+      if (*g) (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+  }
   ~bss_plan() {
+    drop_graphs();
+    if (piv_event) (void)hipEventDestroy(piv_event);
+    if (host_piv) (void)hipHostFree(host_piv);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
   }
 };
@@ -215,6 +256,21 @@ int layout(bss_plan* p) {
         V.orL = bump.take(V.P * nd * fs);
       }
       V.oOrig = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
+      if (mixed && l > 0) {
+        // the FP32 factor keeps its C apart from B, so the reduced couplings survive
+        // for an FP64 re-factorisation
+        V.oC = bump.take(std::max<size_t>(NL - 1, 1) * nn * fs);
+        V.wA = bump.take(NL * nn * 8);
+        V.wB = bump.take(std::max<size_t>(NL - 1, 1) * nn * 8);
+        V.wd = bump.take(NL * nd * 8);
+        if (anyL) V.wW = bump.take(NL * nn * 8);
+        if (V.P > 0) {
+          V.wSR = bump.take(V.P * nn * 8);
+          V.wSL = bump.take(V.P * nn * 8);
+          V.wrR = bump.take(V.P * nd * 8);
+          V.wrL = bump.take(V.P * nd * 8);
+        }
+      }
     }
   }
   // global levels
@@ -239,10 +295,18 @@ int layout(bss_plan* p) {
     GLevel& G = p->gl[l];
     p->toff[l].chains = bump.take(sizeof(bss::ChainRec) * std::max<size_t>(G.chains.size(), 1));
     p->toff[l].seps = bump.take(sizeof(bss::SepRec) * std::max<size_t>(G.seps.size(), 1));
-    for (int k = 0; k < (l == 0 ? 2 : 1); ++k)
+    for (int k = 0; k < (l == 0 ? 2 : 1); ++k) {
       p->toff[l].tab[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
+      if (mixed) p->toff[l].tab64[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
+    }
   }
-  p->info_off = bump.take(sizeof(int) * p->nsys);
+  // info words: [0, nsys) level 0 and FP64 levels, [nsys, 2 nsys) FP32 separator levels
+  p->info_off = bump.take(sizeof(int) * 2 * p->nsys);
+  if (mixed) {
+    p->piv_off = bump.take(sizeof(unsigned) * p->nsys);
+    p->widen_off = bump.take(sizeof(bss::ConvRec<float, double>) * p->nsys);
+    p->narrow_off = bump.take(sizeof(bss::ConvRec<double, float>) * p->nsys);
+  }
   if (mixed) {
     std::vector<int> nu(p->nsys);
     for (int s = 0; s < p->nsys; ++s) nu[s] = p->sp[s].N_use;
@@ -254,7 +318,9 @@ int layout(bss_plan* p) {
 }
 
 // Device pointers of one system at one level; `k` selects the level-0 vector set.
-bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k) {
+// `wide` (mixed mode): levels >= 1 are the FP64 copies; at level 0 only the next-level
+// pointers change (hand-off into FP64 level 1).
+bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k, bool wide = false) {
   const SysPlan& S = p->sp[s];
   const SysLevel& V = S.lv[l];
   auto at = [&](size_t o) -> void* { return o != kNone ? (void*)(p->ws + o) : nullptr; };
@@ -282,22 +348,23 @@ bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k) {
     else
       t.d = p->d ? (char*)p->d + od * is : nullptr;
   } else {
-    t.A = at(V.oA);
-    t.B = at(V.oB);
+    t.A = at(wide ? V.wA : V.oA);
+    t.B = at(wide ? V.wB : V.oB);
     t.U = (void*)t.A;
-    t.C = (void*)t.B;
-    t.d = at(V.od);
+    t.C = (!wide && V.oC != kNone) ? at(V.oC) : (void*)t.B;
+    t.d = at(wide ? V.wd : V.od);
   }
-  t.W = at(V.oW);
-  t.SR = at(V.oSR);
-  t.SL = at(V.oSL);
-  t.rR = at(V.orR);
-  t.rL = at(V.orL);
+  const bool w = wide && l > 0;
+  t.W = at(w ? V.wW : V.oW);
+  t.SR = at(w ? V.wSR : V.oSR);
+  t.SL = at(w ? V.wSL : V.oSL);
+  t.rR = at(w ? V.wrR : V.orR);
+  t.rL = at(w ? V.wrL : V.orL);
   if (l + 1 < S.lv.size()) {
     const SysLevel& Nx = S.lv[l + 1];
-    t.An = at(Nx.oA);
-    t.Bn = at(Nx.oB);
-    t.dn = at(Nx.od);
+    t.An = at(wide ? Nx.wA : Nx.oA);
+    t.Bn = at(wide ? Nx.wB : Nx.oB);
+    t.dn = at(wide ? Nx.wd : Nx.od);
   }
   t.orig = (const int*)at(V.oOrig);
   t.sys = s;
@@ -308,10 +375,12 @@ bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k) {
 int upload_tables(bss_plan* p, size_t l) {
   GLevel& G = p->gl[l];
   for (int k = 0; k < (l == 0 ? 2 : 1); ++k) {
-    std::vector<bss::LevelSys> tab;
-    for (int s : G.slot_sys) tab.push_back(level_sys(p, s, l, k));
-    HIP_TRY(hipMemcpy(G.dtab[k], tab.data(), tab.size() * sizeof(bss::LevelSys),
-                      hipMemcpyHostToDevice));
+    for (int wide = 0; wide < (p->mode == BSS_MODE_MIXED ? 2 : 1); ++wide) {
+      std::vector<bss::LevelSys> tab;
+      for (int s : G.slot_sys) tab.push_back(level_sys(p, s, l, k, wide != 0));
+      HIP_TRY(hipMemcpy(wide ? G.dtab64[k] : G.dtab[k], tab.data(),
+                        tab.size() * sizeof(bss::LevelSys), hipMemcpyHostToDevice));
+    }
   }
   return BSS_OK;
 }
@@ -337,8 +406,11 @@ int build(bss_plan* p) {
     GLevel& G = p->gl[l];
     G.dchains = (bss::ChainRec*)(p->ws + p->toff[l].chains);
     G.dseps = (bss::SepRec*)(p->ws + p->toff[l].seps);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
       G.dtab[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab[l == 0 ? k : 0]);
+      if (p->mode == BSS_MODE_MIXED)
+        G.dtab64[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab64[l == 0 ? k : 0]);
+    }
     if (!G.chains.empty())
       HIP_TRY(hipMemcpy(G.dchains, G.chains.data(), G.chains.size() * sizeof(bss::ChainRec),
                         hipMemcpyHostToDevice));
@@ -360,6 +432,31 @@ int build(bss_plan* p) {
       orig.swap(next);
     }
   }
+  if (p->mode == BSS_MODE_MIXED) {
+    // level-1 couplings FP32 -> FP64 and level-1 vectors FP64 -> FP32, per system
+    p->dpiv = (unsigned*)(p->ws + p->piv_off);
+    p->dwiden = (bss::ConvRec<float, double>*)(p->ws + p->widen_off);
+    p->dnarrow = (bss::ConvRec<double, float>*)(p->ws + p->narrow_off);
+    std::vector<bss::ConvRec<float, double>> wr;
+    std::vector<bss::ConvRec<double, float>> nr;
+    const long long nn = (long long)p->n * p->n, nd = (long long)p->n * p->nrhs;
+    for (int s = 0; s < p->nsys; ++s) {
+      const SysPlan& S = p->sp[s];
+      if (S.lv.size() < 2) continue;
+      const SysLevel& V = S.lv[1];
+      wr.push_back({(const float*)(p->ws + V.oB), (double*)(p->ws + V.wB),
+                    (long long)std::max(V.shape.N - 1, 0) * nn});
+      nr.push_back({(const double*)(p->ws + V.wd), (float*)(p->ws + V.od),
+                    (long long)V.shape.N * nd});
+    }
+    p->nconv = (int)wr.size();
+    HIP_TRY(hipHostMalloc((void**)&p->host_piv, sizeof(unsigned) * 2 * p->nsys));
+    HIP_TRY(hipEventCreateWithFlags(&p->piv_event, hipEventDisableTiming));
+    if (!wr.empty()) {
+      HIP_TRY(hipMemcpy(p->dwiden, wr.data(), wr.size() * sizeof(wr[0]), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(p->dnarrow, nr.data(), nr.size() * sizeof(nr[0]), hipMemcpyHostToDevice));
+    }
+  }
   p->built = true;
   for (size_t l = 1; l < p->gl.size(); ++l)
     if (int e2 = upload_tables(p, l)) return e2;
@@ -369,10 +466,20 @@ int build(bss_plan* p) {
 template <typename Tin, typename T>
 int factor_levels(bss_plan* p, hipStream_t s) {
   const int nb = p->n;
-  HIP_TRY(hipMemsetAsync(p->dinfo, 0x7f, sizeof(int) * p->nsys, s));
+  const bool mixed = p->mode == BSS_MODE_MIXED;
+  hipLaunchKernelGGL(fill_kernel, dim3((2 * p->nsys + 255) / 256), dim3(256), 0, s, p->dinfo,
+                     2 * p->nsys, kInfoNone);
+  if (mixed)
+    hipLaunchKernelGGL(fill_kernel, dim3((p->nsys + 255) / 256), dim3(256), 0, s, (int*)p->dpiv,
+                       p->nsys, kPivNone);
+  HIP_TRY(hipGetLastError());
   for (size_t l = 0; l < p->gl.size(); ++l) {
     GLevel& G = p->gl[l];
-    const bss::FactorLevelArgs a{G.dchains, G.dtab[0], p->dinfo};
+    // mixed mode: FP32 separator levels report into their own info words and feed the
+    // pivot monitor that decides on an FP64 re-factorisation
+    const bss::FactorLevelArgs a{G.dchains, G.dtab[0],
+                                 p->dinfo + ((mixed && l > 0) ? p->nsys : 0),
+                                 (mixed && l > 0) ? p->dpiv : nullptr};
     hipError_t e;
     {
       ProfScope ps(p, BSS_K_FACTOR, (int)l, s);
@@ -385,12 +492,61 @@ int factor_levels(bss_plan* p, hipStream_t s) {
     if (!G.seps.empty()) {
       ProfScope ps(p, BSS_K_ASSEMBLE_DIAG, (int)l, s);
       if (l == 0)
-        e = bss::launch_assemble_diag<Tin, T>(G.dseps, G.dtab[0], nb, (int)G.seps.size(), s);
+        e = bss::launch_assemble_diag<Tin, T, T>(G.dseps, G.dtab[0], nb, (int)G.seps.size(), s);
       else
-        e = bss::launch_assemble_diag<T, T>(G.dseps, G.dtab[0], nb, (int)G.seps.size(), s);
+        e = bss::launch_assemble_diag<T, T, T>(G.dseps, G.dtab[0], nb, (int)G.seps.size(), s);
       if (e != hipSuccess) return fail(BSS_E_HIP, "assemble level %zu: %s", l, hipGetErrorString(e));
     }
   }
+  return BSS_OK;
+}
+
+// Mixed mode: factor the separator levels (>= 1) again in FP64 from the FP64 input
+// separator blocks and the level-0 contributions, into the FP64 copies.
+int refactor_separators64(bss_plan* p, hipStream_t s) {
+  const int nb = p->n;
+  GLevel& G0 = p->gl[0];
+  hipError_t e = bss::launch_assemble_diag<double, float, double>(G0.dseps, G0.dtab64[0], nb,
+                                                                 (int)G0.seps.size(), s);
+  if (e == hipSuccess) e = bss::launch_convert<float, double>(p->dwiden, p->nconv, s);
+  if (e != hipSuccess) return fail(BSS_E_HIP, "FP64 hand-off: %s", hipGetErrorString(e));
+  for (size_t l = 1; l < p->gl.size(); ++l) {
+    GLevel& G = p->gl[l];
+    const bss::FactorLevelArgs a{G.dchains, G.dtab64[0], p->dinfo, nullptr};
+    {
+      ProfScope ps(p, BSS_K_FACTOR, (int)l, s);
+      e = bss::launch_factor<double, double>(nb, p->factor_waves, a, (int)G.chains.size(), s);
+    }
+    if (e == hipSuccess && !G.seps.empty()) {
+      ProfScope ps(p, BSS_K_ASSEMBLE_DIAG, (int)l, s);
+      e = bss::launch_assemble_diag<double, double, double>(G.dseps, G.dtab64[0], nb,
+                                                            (int)G.seps.size(), s);
+    }
+    if (e != hipSuccess) return fail(BSS_E_HIP, "FP64 separator level %zu: %s", l, hipGetErrorString(e));
+  }
+  return BSS_OK;
+}
+
+// Mixed mode, once per factorisation (synchronises s): inspect the FP32 separator
+// pivots and re-factor the separator levels in FP64 when any system's smallest
+// U_jj^2 / A_jj is below the pivot tolerance or an FP32 separator pivot failed.
+int ensure_pivots(bss_plan* p, hipStream_t s) {
+  if (p->mode != BSS_MODE_MIXED || !p->pivots_pending) return BSS_OK;
+  p->pivots_pending = false;
+  p->sep64 = false;
+  p->refactored = 0;
+  if (p->gl.size() < 2) return BSS_OK;
+  HIP_TRY(hipEventSynchronize(p->piv_event));
+  bool need = false;
+  for (int q = 0; q < p->nsys; ++q) {
+    float r;
+    memcpy(&r, &p->host_piv[q], sizeof(r));
+    if (!(r >= (float)p->pivot_tol) || (int)p->host_piv[p->nsys + q] != kInfoNone) need = true;
+  }
+  if (!need) return BSS_OK;
+  if (int e = refactor_separators64(p, s)) return e;
+  p->sep64 = true;
+  p->refactored = (int)p->gl.size() - 1;
   return BSS_OK;
 }
 
@@ -412,7 +568,7 @@ int solve_levels(bss_plan* p, int k, hipStream_t s) {
     if (e != hipSuccess) return fail(BSS_E_HIP, "forward level %zu: %s", l, hipGetErrorString(e));
     if (!G.seps.empty()) {
       ProfScope ps(p, BSS_K_ASSEMBLE_RHS, (int)l, s);
-      e = bss::launch_assemble_rhs<T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
+      e = bss::launch_assemble_rhs<T, T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
       if (e != hipSuccess) return fail(BSS_E_HIP, "rhs assemble %zu: %s", l, hipGetErrorString(e));
     }
   }
@@ -427,10 +583,96 @@ int solve_levels(bss_plan* p, int k, hipStream_t s) {
     if (e != hipSuccess) return fail(BSS_E_HIP, "backward level %zu: %s", li, hipGetErrorString(e));
     if (!G.seps.empty()) {
       ProfScope ps(p, BSS_K_SCATTER, (int)li, s);
-      e = bss::launch_scatter_sep<T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
+      e = bss::launch_scatter_sep<T, T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
       if (e != hipSuccess) return fail(BSS_E_HIP, "scatter %zu: %s", li, hipGetErrorString(e));
     }
   }
+  return BSS_OK;
+}
+
+// Runs `enqueue` on stream s directly, or -- with graphs on and the profiler off --
+// replays a graph captured from it on the plan's private capture stream (the caller's
+// stream may be the legacy null stream, which cannot be captured).  Every kernel
+// argument is a plan-owned address, so one capture stays valid across rebinding.
+template <typename F>
+int run_graph(bss_plan* p, hipGraphExec_t* ex, hipStream_t s, F enqueue) {
+  if (!p->use_graph || p->prof) return enqueue(s);
+  if (!*ex) {
+    if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(p->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc != BSS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(BSS_E_HIP, "graph capture: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      *ex = nullptr;
+      return fail(BSS_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    }
+  }
+  HIP_TRY(hipGraphLaunch(*ex, s));
+  return BSS_OK;
+}
+
+// Mixed-mode FP32 correction solve after an FP64 separator re-factorisation: level 0
+// in FP32 (vector set k), separator levels in FP64, FP32 <-> FP64 at the hand-off.
+int solve_levels_sep64(bss_plan* p, int k, hipStream_t s) {
+  const int nb = p->n;
+  const int nd = p->n * p->nrhs;
+  const size_t L = p->gl.size();
+  GLevel& G0 = p->gl[0];
+  const bss::SolveLevelArgs a0{G0.dchains, G0.dtab[k], p->nrhs};
+  hipError_t e;
+  {
+    ProfScope ps(p, BSS_K_FORWARD, 0, s);
+    e = bss::launch_forward<float>(nb, a0, (int)G0.chains.size(), s);
+  }
+  if (e == hipSuccess) {
+    ProfScope ps(p, BSS_K_ASSEMBLE_RHS, 0, s);
+    e = bss::launch_assemble_rhs<float, double>(G0.dseps, G0.dtab64[k], nd, (int)G0.seps.size(), s);
+  }
+  if (e != hipSuccess) return fail(BSS_E_HIP, "FP32 level of the FP64-separator solve: %s", hipGetErrorString(e));
+  for (size_t l = 1; l < L; ++l) {
+    GLevel& G = p->gl[l];
+    const bss::SolveLevelArgs a{G.dchains, G.dtab64[0], p->nrhs};
+    {
+      ProfScope ps(p, BSS_K_FORWARD, (int)l, s);
+      e = bss::launch_forward<double>(nb, a, (int)G.chains.size(), s);
+    }
+    if (e == hipSuccess && !G.seps.empty()) {
+      ProfScope ps(p, BSS_K_ASSEMBLE_RHS, (int)l, s);
+      e = bss::launch_assemble_rhs<double, double>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
+    }
+    if (e != hipSuccess) return fail(BSS_E_HIP, "FP64 forward level %zu: %s", l, hipGetErrorString(e));
+  }
+  for (size_t l = L - 1; l >= 1; --l) {
+    GLevel& G = p->gl[l];
+    const bss::SolveLevelArgs a{G.dchains, G.dtab64[0], p->nrhs};
+    {
+      ProfScope ps(p, BSS_K_BACKWARD, (int)l, s);
+      e = bss::launch_backward<double>(nb, a, (int)G.chains.size(), s);
+    }
+    if (e == hipSuccess && !G.seps.empty()) {
+      ProfScope ps(p, BSS_K_SCATTER, (int)l, s);
+      e = bss::launch_scatter_sep<double, double>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
+    }
+    if (e != hipSuccess) return fail(BSS_E_HIP, "FP64 backward level %zu: %s", l, hipGetErrorString(e));
+  }
+  e = bss::launch_convert<double, float>(p->dnarrow, p->nconv, s);
+  if (e == hipSuccess) {
+    ProfScope ps(p, BSS_K_BACKWARD, 0, s);
+    e = bss::launch_backward<float>(nb, a0, (int)G0.chains.size(), s);
+  }
+  if (e == hipSuccess) {
+    ProfScope ps(p, BSS_K_SCATTER, 0, s);
+    e = bss::launch_scatter_sep<float, float>(G0.dseps, a0.sys, nd, (int)G0.seps.size(), s);
+  }
+  if (e != hipSuccess) return fail(BSS_E_HIP, "FP32 back-substitution: %s", hipGetErrorString(e));
   return BSS_OK;
 }
 
@@ -531,6 +773,11 @@ int bss_set_option(bss_plan_t p, int option, double value) {
     case BSS_OPT_FACTOR_WAVES:
       if (value != 4 && value != 8) return fail(BSS_E_ARG, "factor waves must be 4 or 8");
       p->factor_waves = (int)value;
+      p->drop_graphs();
+      return BSS_OK;
+    case BSS_OPT_GRAPH:
+      p->use_graph = value != 0.0;
+      if (!p->use_graph) p->drop_graphs();
       return BSS_OK;
     case BSS_OPT_BASE_MAX:
       if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "base_max must be >= 1");
@@ -622,12 +869,24 @@ int bss_factorize(bss_plan_t p, void* stream) {
   p->last_stream = s;
   int rc;
   if (p->mode == BSS_MODE_MIXED) {
-    rc = factor_levels<double, float>(p, s);
-    if (rc == BSS_OK) bss::mixed_after_factor(p->mx);
+    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, float>(p, c); });
+    if (rc == BSS_OK && p->gl.size() > 1) {
+      // pivot ratios and FP32 separator info to pinned memory; read at the first solve
+      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dpiv, sizeof(unsigned) * p->nsys,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(p->host_piv + p->nsys, p->dinfo + p->nsys, sizeof(int) * p->nsys,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipEventRecord(p->piv_event, s));
+    }
+    if (rc == BSS_OK) {
+      bss::mixed_after_factor(p->mx);
+      p->pivots_pending = true;
+      p->sep64 = false;
+    }
   } else if (p->dtype == BSS_F64) {
-    rc = factor_levels<double, double>(p, s);
+    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, double>(p, c); });
   } else {
-    rc = factor_levels<float, float>(p, s);
+    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<float, float>(p, c); });
   }
   if (rc == BSS_OK) p->factored = true;
   return rc;
@@ -641,8 +900,13 @@ int bss_solve(bss_plan_t p, void* stream) {
   if (p->mode == BSS_MODE_MIXED) {
     bss::MixedSolveHooks hooks;
     hooks.ctx = p;
+    hooks.prepare = [](void* ctx, hipStream_t st) -> int { return ensure_pivots((bss_plan*)ctx, st); };
     hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
-      return solve_levels<float>((bss_plan*)ctx, k, st);
+      bss_plan* q = (bss_plan*)ctx;
+      if (q->sep64)
+        return run_graph(q, &q->g_solve64[k], st,
+                         [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
+      return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
     };
     hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
       return new ProfScope((bss_plan*)ctx, kind, 0, st);
@@ -654,18 +918,25 @@ int bss_solve(bss_plan_t p, void* stream) {
     if (rc != 0) return fail(BSS_E_HIP, "mixed solve failed: %s", bss::mixed_last_error());
     return BSS_OK;
   }
-  if (p->dtype == BSS_F64) return solve_levels<double>(p, 0, s);
-  return solve_levels<float>(p, 0, s);
+  if (p->dtype == BSS_F64)
+    return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<double>(p, 0, c); });
+  return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<float>(p, 0, c); });
 }
 
 int bss_info(bss_plan_t p, int* info_host) {
   if (int e = check_plan(p)) return e;
   if (!info_host) return fail(BSS_E_ARG, "info_host is null");
   if (!p->built || !p->factored) return fail(BSS_E_STATE, "no factorization yet");
+  if (int e = ensure_pivots(p, p->last_stream)) return e;
   HIP_TRY(hipStreamSynchronize(p->last_stream));
-  HIP_TRY(hipMemcpy(info_host, p->dinfo, sizeof(int) * p->nsys, hipMemcpyDeviceToHost));
-  for (int s = 0; s < p->nsys; ++s)
-    if (info_host[s] == kInfoNone) info_host[s] = 0;
+  std::vector<int> info(2 * p->nsys);
+  HIP_TRY(hipMemcpy(info.data(), p->dinfo, sizeof(int) * 2 * p->nsys, hipMemcpyDeviceToHost));
+  for (int s = 0; s < p->nsys; ++s) {
+    // FP32 separator failures count unless the separator levels were re-factored in FP64
+    int v = info[s];
+    if (p->mode == BSS_MODE_MIXED && !p->sep64) v = std::min(v, info[p->nsys + s]);
+    info_host[s] = v == kInfoNone ? 0 : v;
+  }
   return BSS_OK;
 }
 

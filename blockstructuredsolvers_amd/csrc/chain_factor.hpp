@@ -36,6 +36,8 @@ struct FactorLevelArgs {
   const ChainRec* chains;  // one workgroup per chain
   const LevelSys* sys;     // per-system buffers of this level
   int* info;               // per system, atomicMin of the 1-based failing block index
+  unsigned* pivmin;        // per system, atomicMin of the float bits of min_j U_jj^2 / A_jj
+                           // over the level's blocks (null: not monitored)
 };
 
 template <typename T>
@@ -289,11 +291,27 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs p
     if (!last) sa.load(A + (long long)(blk + 1) * NN);
     if (i + 1 < len - 1 || (i + 1 == len - 1 && hasR)) sb.load(B + (long long)(blk + 1) * NN);
 
+    // pivot monitor: wave 0, lane j < NB keeps the diagonal entry A_jj of this block
+    float adiag = 0.0f;
+    if (p.pivmin && w == 0 && lane_id() < NB) adiag = (float)Abuf[lane_id() * (LD + 1)];
     potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
     if (tid == 0 && fail) {
       const int ob = S.orig ? S.orig[blk] : blk;
       atomicMin(p.info + sys, ob + 1);
       fail = 0;
+    }
+    if (p.pivmin && w == 0) {
+      // U_jj^2 / A_jj = 1 / (Uinv_jj^2 A_jj): the fraction of the diagonal left after
+      // elimination; a failed or non-finite pivot counts as 0
+      float ratio = 1.0f;
+      if (lane_id() < NB) {
+        const float ui = (float)Ubuf[lane_id() * (LD + 1)];
+        const float q = 1.0f / (ui * ui * adiag);
+        ratio = (q >= 0.0f && q <= 3.0e38f) ? q : 0.0f;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) ratio = fminf(ratio, __shfl_xor(ratio, o));
+      if (lane_id() == 0) atomicMin(p.pivmin + sys, __float_as_uint(ratio));
     }
 
     // [C | W] = Uinv^T [B | Z].  Output tile (r, c) costs r+1 tile products; wave w

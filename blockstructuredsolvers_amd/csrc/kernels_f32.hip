@@ -5,7 +5,7 @@ namespace bss {
 template hipError_t launch_factor<float, float>(int, int, const FactorLevelArgs&, int, hipStream_t);
 template hipError_t launch_forward<float>(int, const SolveLevelArgs&, int, hipStream_t);
 template hipError_t launch_backward<float>(int, const SolveLevelArgs&, int, hipStream_t);
-template hipError_t launch_assemble_diag<float, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
-template hipError_t launch_assemble_rhs<float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
-template hipError_t launch_scatter_sep<float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
+template hipError_t launch_assemble_diag<float, float, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
+template hipError_t launch_assemble_rhs<float, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
+template hipError_t launch_scatter_sep<float, float>(const SepRec*, const LevelSys*, int, int, hipStream_t);
 }  // namespace bss

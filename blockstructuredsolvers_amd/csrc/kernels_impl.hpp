@@ -55,28 +55,28 @@ hipError_t launch_backward(int nb, const SolveLevelArgs& a, int nchains, hipStre
   }
 }
 
-template <typename Tin, typename T>
+template <typename Tin, typename Ts, typename To>
 hipError_t launch_assemble_diag(const SepRec* seps, const LevelSys* sys, int nb, int nsep,
                                 hipStream_t s) {
   if (nsep <= 0) return hipSuccess;
-  hipLaunchKernelGGL((assemble_diag_kernel<Tin, T>), dim3(nsep), dim3(256), 0, s, seps, sys,
+  hipLaunchKernelGGL((assemble_diag_kernel<Tin, Ts, To>), dim3(nsep), dim3(256), 0, s, seps, sys,
                      nb * nb);
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename Ts, typename To>
 hipError_t launch_assemble_rhs(const SepRec* seps, const LevelSys* sys, int nd, int nsep,
                                hipStream_t s) {
   if (nsep <= 0) return hipSuccess;
-  hipLaunchKernelGGL((assemble_rhs_kernel<T>), dim3(nsep), dim3(256), 0, s, seps, sys, nd);
+  hipLaunchKernelGGL((assemble_rhs_kernel<Ts, To>), dim3(nsep), dim3(256), 0, s, seps, sys, nd);
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename Ts, typename To>
 hipError_t launch_scatter_sep(const SepRec* seps, const LevelSys* sys, int nd, int nsep,
                               hipStream_t s) {
   if (nsep <= 0) return hipSuccess;
-  hipLaunchKernelGGL((scatter_sep_kernel<T>), dim3(nsep), dim3(256), 0, s, seps, sys, nd);
+  hipLaunchKernelGGL((scatter_sep_kernel<Ts, To>), dim3(nsep), dim3(256), 0, s, seps, sys, nd);
   return hipGetLastError();
 }
 

@@ -17,14 +17,17 @@ template <typename T>
 hipError_t launch_forward(int nb, const SolveLevelArgs& a, int nchains, hipStream_t s);
 template <typename T>
 hipError_t launch_backward(int nb, const SolveLevelArgs& a, int nchains, hipStream_t s);
-template <typename Tin, typename T>
+// Ts: type of this level's contributions/vectors; To: type of the next level.
+template <typename Tin, typename Ts, typename To>
 hipError_t launch_assemble_diag(const SepRec* seps, const LevelSys* sys, int nb, int nsep,
                                 hipStream_t s);
-template <typename T>
+template <typename Ts, typename To>
 hipError_t launch_assemble_rhs(const SepRec* seps, const LevelSys* sys, int nd, int nsep,
                                hipStream_t s);
-template <typename T>
+template <typename Ts, typename To>
 hipError_t launch_scatter_sep(const SepRec* seps, const LevelSys* sys, int nd, int nsep,
                               hipStream_t s);
+template <typename S, typename D>
+hipError_t launch_convert(const ConvRec<S, D>* recs, int nrec, hipStream_t s);
 
 }  // namespace bss

@@ -5,7 +5,7 @@
 // src/backbone_rocm.jl:134-136, 152-153).  Here each chain writes its contributions
 // to per-separator slots (two writers per separator would otherwise race), and these
 // kernels combine them in a fixed order, so results do not depend on scheduling:
-//   A'_p = A[sep_p] - SR_p - SL_p         (reduced diagonal)
+//   A'_p = A[sep_p] - SR_p - SL_p         (reduced diagonal, combined in FP64)
 //   d'_p = d[sep_p] - rR_p - rL_p         (reduced right-hand side)
 //   d[sep_p] = x'_p                       (separator solution back into the level)
 #pragma once
@@ -15,45 +15,63 @@
 
 namespace bss {
 
-// one workgroup per separator (all systems of the level in one launch)
-template <typename Tin, typename T>
+// One workgroup per separator (all systems of the level in one launch).  Ts is the
+// element type of the chain contributions and of this level's vectors, To that of the
+// next level; they differ only at the hand-off from an FP32 level to FP64 separator
+// levels (mixed mode after a separator re-factorisation).
+template <typename Tin, typename Ts, typename To>
 __global__ void __launch_bounds__(256) assemble_diag_kernel(const SepRec* __restrict__ seps,
                                                              const LevelSys* __restrict__ sys,
                                                              int nn) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
   const Tin* a = (const Tin*)S.A + (long long)sr.pos * nn;
-  const T* r = (const T*)S.SR + (long long)sr.p * nn;
-  const T* l = (const T*)S.SL + (long long)sr.p * nn;
-  T* an = (T*)S.An + (long long)sr.p * nn;
+  const Ts* r = (const Ts*)S.SR + (long long)sr.p * nn;
+  const Ts* l = (const Ts*)S.SL + (long long)sr.p * nn;
+  To* an = (To*)S.An + (long long)sr.p * nn;
   for (int e = threadIdx.x; e < nn; e += blockDim.x) {
     const double v = (double)a[e] - (double)r[e] - (double)l[e];
-    an[e] = (T)v;
+    an[e] = (To)v;
   }
 }
 
-template <typename T>
+template <typename Ts, typename To>
 __global__ void __launch_bounds__(256) assemble_rhs_kernel(const SepRec* __restrict__ seps,
                                                             const LevelSys* __restrict__ sys,
                                                             int nd) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
-  const T* src = (const T*)S.d + (long long)sr.pos * nd;
-  const T* a = (const T*)S.rR + (long long)sr.p * nd;
-  const T* b = (const T*)S.rL + (long long)sr.p * nd;
-  T* dst = (T*)S.dn + (long long)sr.p * nd;
-  for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = src[e] - a[e] - b[e];
+  const Ts* src = (const Ts*)S.d + (long long)sr.pos * nd;
+  const Ts* a = (const Ts*)S.rR + (long long)sr.p * nd;
+  const Ts* b = (const Ts*)S.rL + (long long)sr.p * nd;
+  To* dst = (To*)S.dn + (long long)sr.p * nd;
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+    if constexpr (sizeof(Ts) == sizeof(To))
+      dst[e] = src[e] - a[e] - b[e];
+    else
+      dst[e] = (To)((double)src[e] - (double)a[e] - (double)b[e]);
+  }
 }
 
-template <typename T>
+template <typename Ts, typename To>
 __global__ void __launch_bounds__(256) scatter_sep_kernel(const SepRec* __restrict__ seps,
                                                            const LevelSys* __restrict__ sys,
                                                            int nd) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
-  const T* src = (const T*)S.dn + (long long)sr.p * nd;
-  T* dst = (T*)S.d + (long long)sr.pos * nd;
-  for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = src[e];
+  const To* src = (const To*)S.dn + (long long)sr.p * nd;
+  Ts* dst = (Ts*)S.d + (long long)sr.pos * nd;
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = (Ts)src[e];
+}
+
+// Element-type conversion of whole buffers (one ConvRec per buffer, grid-stride in
+// blockIdx.y).
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) convert_kernel(const ConvRec<S, D>* __restrict__ recs) {
+  const ConvRec<S, D> r = recs[blockIdx.x];
+  for (long long e = (long long)blockIdx.y * blockDim.x + threadIdx.x; e < r.count;
+       e += (long long)gridDim.y * blockDim.x)
+    r.dst[e] = (D)r.src[e];
 }
 
 }  // namespace bss

@@ -211,28 +211,30 @@ hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, i
   return hipGetLastError();
 }
 
-hipError_t residual(const MixedState& m, bool init, int xo_k, int dx_k, int xn_k, int ro_k,
-                    int want_mnorm, hipStream_t s) {
-  hipError_t e;
-  switch (m.n) {
-    case 16: e = launch_residual<16>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
-    case 32: e = launch_residual<32>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
-    case 48: e = launch_residual<48>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
-    case 64: e = launch_residual<64>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
-    default: return hipErrorInvalidValue;
-  }
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(kRedThreads), 0, s, m.sys_dev, m.part,
-                     m.mnorm, m.dnorm, m.berr, m.nrhs, init ? 1 : 0, want_mnorm);
-  return hipGetLastError();
-}
+// profiler kinds (enum bss_kernel_kind in bss.h)
+constexpr int kKindResidual = 6, kKindResidualInit = 7, kKindBerr = 8;
 
-constexpr int kKindResidual = 6;  // BSS_K_RESIDUAL in bss.h
-
+// One refinement pass: the residual kernel, then the per-system backward error.
 hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, bool init, int xo_k,
                          int dx_k, int xn_k, int ro_k, int want_mnorm, hipStream_t s) {
-  void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindResidual, s) : nullptr;
-  hipError_t e = residual(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s);
+  hipError_t e;
+  {
+    void* scope = h.prof_begin ? h.prof_begin(h.ctx, init ? kKindResidualInit : kKindResidual, s)
+                               : nullptr;
+    switch (m.n) {
+      case 16: e = launch_residual<16>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      case 32: e = launch_residual<32>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      case 48: e = launch_residual<48>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      case 64: e = launch_residual<64>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      default: e = hipErrorInvalidValue;
+    }
+    if (scope) h.prof_end(scope);
+  }
+  if (e != hipSuccess) return e;
+  void* scope = h.prof_begin ? h.prof_begin(h.ctx, kKindBerr, s) : nullptr;
+  hipLaunchKernelGGL(berr_kernel, dim3(m.nsys), dim3(kRedThreads), 0, s, m.sys_dev, m.part,
+                     m.mnorm, m.dnorm, m.berr, m.nrhs, init ? 1 : 0, want_mnorm);
+  e = hipGetLastError();
   if (scope) h.prof_end(scope);
   return e;
 }
@@ -335,6 +337,10 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
   hipError_t e = residual_prof(m, h, true, -1, -1, 0, 0, 0, s);
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
+    return -1;
+  }
+  if (h.prepare && h.prepare(h.ctx, s) != 0) {
+    g_mixed_err = "solve preparation failed";
     return -1;
   }
   while (true) {

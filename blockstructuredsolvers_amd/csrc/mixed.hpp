@@ -60,7 +60,10 @@ struct MixedSolveHooks {
   void* ctx = nullptr;
   // FP32 recursive solve in place in the level-0 vectors of ping-pong set k
   int (*solve32)(void* ctx, int k, hipStream_t s) = nullptr;
-  // optional profiler brackets around the residual launches (kind = BSS_K_RESIDUAL)
+  // optional, called once after the first (x = 0) residual pass is enqueued and before
+  // the first solve32: the caller may wait on earlier work and choose the solve path
+  int (*prepare)(void* ctx, hipStream_t s) = nullptr;
+  // optional profiler brackets around the residual and backward-error launches
   void* (*prof_begin)(void* ctx, int kind, hipStream_t s) = nullptr;
   void (*prof_end)(void* scope) = nullptr;
 };

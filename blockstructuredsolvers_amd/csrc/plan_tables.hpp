@@ -44,4 +44,14 @@ struct LevelSys {
   int N;            // blocks at this level
 };
 
+// One buffer to convert element-wise S -> D (count elements): the reduced couplings
+// FP32 -> FP64 when separator levels are re-factored in FP64, and the separator
+// solutions FP64 -> FP32 on their way back to an FP32 level.
+template <typename S, typename D>
+struct ConvRec {
+  const S* src;
+  D* dst;
+  long long count;
+};
+
 }  // namespace bss

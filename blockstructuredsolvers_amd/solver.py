@@ -46,7 +46,7 @@ def _stream_handle(device) -> int:
 
 
 def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                 tol, max_iters) -> None:
+                 tol, max_iters, graph=False, pivot_tol=None) -> None:
     if chain_len is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
     if base_max is not None:
@@ -62,6 +62,10 @@ def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, pres
         L.call("bss_set_option", plan, L.BSS_OPT_TOL, float(tol))
     if max_iters is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_MAX_ITERS, float(max_iters))
+    if graph:
+        L.call("bss_set_option", plan, L.BSS_OPT_GRAPH, 1.0)
+    if pivot_tol is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_PIVOT_TOL, float(pivot_tol))
 
 
 class BlockTriDiagData_batched:
@@ -74,7 +78,8 @@ class BlockTriDiagData_batched:
                  preserve_input: bool = False, tol: Optional[float] = None,
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
-                 factor_waves: Optional[int] = None):
+                 factor_waves: Optional[int] = None, graph: bool = False,
+                 pivot_tol: Optional[float] = None):
         self.n_user = N
         self.n = n
         self.nrhs = nrhs
@@ -91,7 +96,7 @@ class BlockTriDiagData_batched:
                    ctypes.byref(plan))
         self._plan = plan
         _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                     tol, max_iters)
+                     tol, max_iters, graph, pivot_tol)
         self.N = self._get(L.BSS_Q_N_USE)
         self.dtype = _torch_dtype(code)
         NU = self.N
@@ -159,6 +164,12 @@ class BlockTriDiagData_batched:
     def iterations(self) -> int:
         return self._get(L.BSS_Q_LAST_ITERS)
 
+    @property
+    def refactored(self) -> int:
+        """Mixed mode: separator levels re-factored in FP64 after the last factorize
+        (known once a solve or info() has inspected the FP32 pivots)."""
+        return self._get(L.BSS_Q_REFACTORED)
+
     def synchronize(self) -> None:
         with torch.cuda.device(self.device):
             L.call("bss_sync", self._plan)
@@ -190,7 +201,8 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
                  preserve_input: bool = False, tol: Optional[float] = None,
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
-                 factor_waves: Optional[int] = None):
+                 factor_waves: Optional[int] = None, graph: bool = False,
+                 pivot_tol: Optional[float] = None):
         Ns = [int(v) for v in Ns]
         self.Ns_user = Ns
         self.n = n
@@ -209,7 +221,7 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
                    sched_code, ctypes.byref(plan))
         self._plan = plan
         _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                     tol, max_iters)
+                     tol, max_iters, graph, pivot_tol)
         nu = (ctypes.c_int * len(Ns))()
         L.call("bss_plan_system_sizes", plan, nu)
         self.N_use = [int(v) for v in nu]
@@ -246,7 +258,8 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
                        max_iters: Optional[int] = None,
                        chain_len_levels: Optional[List[int]] = None,
                        factor_waves: Optional[int] = None,
-                       tuned: bool = True) -> BlockTriDiagData_batched:
+                       tuned: bool = True, graph: bool = False,
+                       pivot_tol: Optional[float] = None) -> BlockTriDiagData_batched:
     """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
 
     ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
@@ -257,7 +270,10 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     ``preserve_input`` keeps A and B unchanged in plain mode (factors go to the plan's
     workspace); ``tol``/``max_iters`` control mixed-mode refinement; ``chain_len``,
     ``chain_len_levels``, ``base_max`` and ``factor_waves`` fix the auto schedule,
-    otherwise (``tuned=True``) it comes from the autotuner cache when one exists."""
+    otherwise (``tuned=True``) it comes from the autotuner cache when one exists;
+    ``graph=True`` replays each launch sequence from a captured hipGraph;
+    ``pivot_tol`` (mixed) is the FP32 separator pivot threshold U_jj^2 / A_jj below
+    which the separator levels are re-factored in FP64."""
     if not torch.cuda.is_available():
         raise RuntimeError("initialize_batched needs a HIP device; none is visible")
     device = torch.device(M) if not isinstance(M, torch.device) else M
@@ -274,7 +290,7 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
             factor_waves = cfg.get("factor_waves")
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
                                     base_max, preserve_input, tol, max_iters,
-                                    chain_len_levels, factor_waves)
+                                    chain_len_levels, factor_waves, graph, pivot_tol)
 
 
 def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *,
@@ -284,7 +300,8 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
                              tol: Optional[float] = None, max_iters: Optional[int] = None,
                              chain_len_levels: Optional[List[int]] = None,
                              factor_waves: Optional[int] = None,
-                             tuned: bool = True) -> BlockTriDiagData_group:
+                             tuned: bool = True, graph: bool = False,
+                             pivot_tol: Optional[float] = None) -> BlockTriDiagData_group:
     """Uneven batch: like ``initialize_batched`` but system s has ``Ns[s]`` blocks.
     Schedule options apply to every system.  With ``tuned=True`` the autotuner cache is
     used only when it names one configuration for all the sizes in ``Ns``, so each
@@ -306,7 +323,7 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
             factor_waves = cfgs[0].get("factor_waves")
     return BlockTriDiagData_group(list(Ns), n, T, device, nrhs, mode, schedule, chain_len,
                                   base_max, preserve_input, tol, max_iters, chain_len_levels,
-                                  factor_waves)
+                                  factor_waves, graph, pivot_tol)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:

@@ -73,15 +73,21 @@ enum bss_query {
 enum bss_option {
   BSS_OPT_TOL = 0,          /* mixed: target normwise backward error (default 1e-12) */
   BSS_OPT_MAX_ITERS = 1,    /* mixed: max refinement iterations (default 10) */
-  BSS_OPT_PIVOT_TOL = 2,    /* mixed: FP32 pivot threshold relative to the diagonal
-                               (default 1e-5); below it the separator levels are
-                               re-factored in FP64 */
+  BSS_OPT_PIVOT_TOL = 2,    /* mixed: threshold on U_jj^2 / A_jj of the FP32 factorisation
+                               of the separator levels (default 1e-5).  When a system
+                               falls below it, or an FP32 separator pivot fails, the
+                               separator levels are re-factored in FP64 at the next
+                               bss_solve or bss_info (which synchronise) */
   BSS_OPT_CHAIN_LEN = 3,    /* auto schedule: chain length for every level */
   BSS_OPT_BASE_MAX = 4,     /* auto schedule: largest single-chain base level */
   BSS_OPT_PRESERVE_INPUT = 5, /* plain mode: 1 = keep A and B unchanged, factors go to
                                the plan's workspace (default 0: overwrite, as the
                                reference does) */
   BSS_OPT_FACTOR_WAVES = 6,  /* waves per factor workgroup at n = 64: 4 (default) or 8 */
+  BSS_OPT_GRAPH = 7,         /* 1 = capture each launch sequence (factorize; solve; in
+                               mixed mode each FP32 correction solve) into a hipGraph on
+                               first use and replay it afterwards (default 0).  Ignored
+                               while the profiler is on. */
   BSS_OPT_CHAIN_LEN_LEVEL = 32 /* + level k (< 16): chain length of level k only
                                   (auto schedule); overrides BSS_OPT_CHAIN_LEN there */
 };
@@ -94,9 +100,10 @@ enum bss_kernel_kind {
   BSS_K_ASSEMBLE_RHS = 3,  /* reduced right-hand side assembly */
   BSS_K_BACKWARD = 4,      /* backward substitution sweep */
   BSS_K_SCATTER = 5,       /* separator solution scatter */
-  BSS_K_RESIDUAL = 6,      /* FP64 residual + update (mixed mode) */
-  BSS_K_MNORM = 7,         /* reserved: ||M||_inf is folded into the first residual */
-  BSS_K_COUNT = 8
+  BSS_K_RESIDUAL = 6,      /* FP64 solution update + residual (mixed mode) */
+  BSS_K_RESIDUAL_INIT = 7, /* first refinement pass: x = 0, r = d (mixed mode) */
+  BSS_K_BERR = 8,          /* per-system backward-error reduction (mixed mode) */
+  BSS_K_COUNT = 9
 };
 
 /* Create a plan for `nsys` independent systems of N blocks of size n with `nrhs`

@@ -8,7 +8,7 @@ Two restatements of the reference live here:
 
 * ``factorize`` / ``solve`` -- ctypes bindings of ``bss_oracle.c``, the C
   restatement of the sequential block Cholesky of ``src/cpu.jl`` +
-  ``src/backbone_cpu.jl:5-33`` (the FP64 oracle the north star names).
+  ``src/backbone_cpu.jl:5-33`` (the FP64 block-Cholesky path of the reference).
 * ``ref_build_sequence`` / ``ref_find_sequence_upper`` -- the reference's integer
   level-schedule search, ``src/gpu_batched.jl:112-162``, and
   ``ref_batched_factorize`` / ``ref_batched_solve`` -- a small numpy restatement of

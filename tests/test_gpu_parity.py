@@ -1,7 +1,7 @@
 """GPU parity of the libbss solver against the FP64 CPU oracle (restatement of the
 reference's cpu.jl path) on identical seeded inputs.
 
-Bars (the north star's): plain FP64 -- relative solution difference <= 1e-12 and
+Bars: plain FP64 -- relative solution difference <= 1e-12 and
 normwise backward error <= 1e-13; plain FP32 -- the reference tests' bound
 ||x - x_ref|| <= 10 sqrt(eps32) ||x_ref|| (test/test_rocm.jl:118); mixed -- backward
 error <= 1e-12.
@@ -238,3 +238,78 @@ def test_group_uneven_mixed_and_info():
     _, data = run_gpu_group(systems, tuned=False)
     info = data.info()
     assert info[0] == 0 and info[2] == 0 and info[1] > 0
+
+
+@pytest.mark.parametrize("mode,T", [("plain", torch.float64), ("plain", torch.float32),
+                                    ("mixed", torch.float64)])
+def test_graph_replay_matches_eager(mode, T):
+    """With hipGraph replay on, every replay (new A, B and d each time, with unrelated
+    eager device work and synchronisations between replays) is bitwise the eager
+    result of the same plan configuration."""
+    N, n = 300, 32
+    plans = {g: initialize_batched(N, n, T, "cuda", nrhs=2, mode=mode, tuned=False, graph=g)
+             for g in (False, True)}
+    junk = torch.randn(512, 512, device="cuda")
+    for rep in range(4):
+        sysm = workloads.spd_system(N, n, nrhs=2, seed=500 + rep)
+        out = {}
+        for g, data in plans.items():
+            data.A_tensor.copy_(torch.from_numpy(sysm.A))
+            data.B_tensor.copy_(torch.from_numpy(sysm.B))
+            data.d_tensor.copy_(torch.from_numpy(sysm.d))
+            factorize(data)
+            solve(data)
+            junk = torch.tanh(junk @ junk.T * 1e-3)  # eager work between replays
+            torch.cuda.synchronize()
+            out[g] = data.d_tensor.double().cpu().numpy()
+        assert np.array_equal(out[True], out[False]), rep
+        if T == torch.float64:
+            bound = 1e-13 if mode == "plain" else 1e-12
+            assert oracle.backward_error(sysm.A, sysm.B, out[True], sysm.d).max() <= bound
+    # repeated solves on the same factors replay the solve graph only
+    data = plans[True]
+    d2 = np.asfortranarray(np.random.default_rng(9).uniform(size=(n, 2, N)))
+    for _ in range(3):
+        data.d_tensor.copy_(torch.from_numpy(d2))
+        solve(data)
+        torch.cuda.synchronize()
+    x2 = data.d_tensor.double().cpu().numpy()
+    plans[False].d_tensor.copy_(torch.from_numpy(d2))
+    solve(plans[False])
+    torch.cuda.synchronize()
+    assert np.array_equal(x2, plans[False].d_tensor.double().cpu().numpy())
+
+
+@pytest.mark.parametrize("n,nrhs", [(32, 2), (64, 1), (16, 16)])
+def test_mixed_fp64_separator_refactor(n, nrhs):
+    """Mixed mode re-factors the separator levels in FP64 when an FP32 separator pivot
+    falls below the threshold; a threshold above every possible U_jj^2 / A_jj (2.0)
+    forces that path.  The refined answer meets the same bars, and a well-conditioned
+    input under the default threshold keeps FP32 separators."""
+    sysm = workloads.spd_system(700, n, nrhs=nrhs, seed=600 + n)
+    x, data = run_gpu(sysm, mode="mixed", pivot_tol=2.0, tuned=False)
+    assert data.levels > 1 and data.refactored == data.levels - 1
+    assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
+    xo, _ = oracle.solve_system(sysm.A, sysm.B, sysm.d)
+    assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+    assert data.info()[0] == 0
+    it64 = data.iterations
+    # repeated solve on the FP64-separator factors
+    d2 = np.asfortranarray(np.random.default_rng(4).uniform(size=sysm.d.shape))
+    data.d_tensor.copy_(torch.from_numpy(d2))
+    solve(data)
+    torch.cuda.synchronize()
+    assert oracle.backward_error(sysm.A, sysm.B, data.d_tensor.cpu().numpy(), d2).max() <= 1e-12
+    x32, d32 = run_gpu(sysm, mode="mixed", tuned=False)
+    assert d32.refactored == 0
+    assert it64 <= d32.iterations
+
+
+def test_mixed_fp64_separator_refactor_group_graph():
+    """The FP64-separator path in an uneven group with graph replay."""
+    systems = [workloads.spd_system(N, 32, nrhs=1, seed=700 + N) for N in (2, 90, 600)]
+    for graph in (False, True):
+        xs, data = run_gpu_group(systems, mode="mixed", pivot_tol=2.0, tuned=False, graph=graph)
+        assert data.refactored == data.levels - 1
+        for sysm, x in zip(systems, xs):
+            assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12

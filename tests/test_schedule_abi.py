@@ -117,3 +117,56 @@ def test_reference_schedule_levels_layout():
     assert len(chains) == 11
     assert all(c == (k * 9 + 1, 8, 1, 1) for k, c in enumerate(chains))
     assert levels[1][1] == [(0, 12, 0, 0)]
+
+
+def test_group_plan_sizes_and_levels():
+    """An uneven group: each system keeps the schedule it has alone (same level sizes),
+    the group has as many levels as its deepest system and one workgroup per chain of
+    every system that has a level."""
+    Ns = [1, 7, 100, 1000, 8192]
+    p = ctypes.c_void_p()
+    arr = (ctypes.c_int * len(Ns))(*Ns)
+    L.call("bss_plan_create_group", len(Ns), arr, 64, 1, 1, 0, 0, ctypes.byref(p))
+    try:
+        nu = (ctypes.c_int * len(Ns))()
+        assert L.call("bss_plan_system_sizes", p, nu) == len(Ns)
+        assert list(nu) == Ns
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_LEVELS, ctypes.byref(v))
+        levels = v.value
+        alone = [plan_chains(N) for N in Ns]
+        assert levels == max(len(a[1]) for a in alone)
+        for k in range(levels):
+            L.call("bss_plan_get", p, L.BSS_Q_LEVEL_CHAINS + k, ctypes.byref(v))
+            assert v.value == sum(len(a[1][k][1]) for a in alone if k < len(a[1]))
+    finally:
+        L.lib().bss_plan_destroy(p)
+    # reference schedule pads each system on its own
+    L.call("bss_plan_create_group", len(Ns), arr, 32, 1, 1, 0, 1, ctypes.byref(p))
+    try:
+        nu = (ctypes.c_int * len(Ns))()
+        L.call("bss_plan_system_sizes", p, nu)
+        assert list(nu) == [bss.reference_schedule(N)[0] for N in Ns]
+    finally:
+        L.lib().bss_plan_destroy(p)
+
+
+def test_group_argument_errors_and_options():
+    p = ctypes.c_void_p()
+    bad = (ctypes.c_int * 3)(5, 0, 7)
+    assert L.lib().bss_plan_create_group(3, bad, 32, 1, 1, 0, 0, ctypes.byref(p)) == -1
+    assert "system 1" in L.last_error()
+    assert L.lib().bss_plan_create_group(0, bad, 32, 1, 1, 0, 0, ctypes.byref(p)) == -1
+    ok = (ctypes.c_int * 2)(5, 9)
+    L.call("bss_plan_create_group", 2, ok, 32, 1, 1, 1, 0, ctypes.byref(p))
+    try:
+        L.call("bss_set_option", p, L.BSS_OPT_GRAPH, 1.0)
+        L.call("bss_set_option", p, L.BSS_OPT_PIVOT_TOL, 2.0)
+        assert L.lib().bss_set_option(p, L.BSS_OPT_PIVOT_TOL, -1.0) == -1
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_WORKSPACE_BYTES, ctypes.byref(v))
+        assert v.value > 0
+        # schedule options are frozen once the layout exists
+        assert L.lib().bss_set_option(p, L.BSS_OPT_CHAIN_LEN, 2.0) == -4
+    finally:
+        L.lib().bss_plan_destroy(p)
