@@ -47,6 +47,7 @@ SIGNATURES = {
     "bss_plan_system_sizes": ([_VP, ctypes.POINTER(_I)], _I),
     "bss_set_option": ([_VP, _I, _D], _I),
     "bss_plan_get": ([_VP, _I, ctypes.POINTER(ctypes.c_int64)], _I),
+    "bss_plan_set_workspace": ([_VP, _VP, ctypes.c_size_t], _I),
     "bss_plan_bind": ([_VP, _VP, _VP, _VP], _I),
     "bss_factorize": ([_VP, _VP], _I),
     "bss_solve": ([_VP, _VP], _I),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -98,6 +99,7 @@ struct bss_plan {
   std::vector<GLevel> gl;
   char* ws = nullptr;
   size_t ws_bytes = 0;
+  bool ws_owned = true;  // false: caller's memory (bss_plan_set_workspace)
   void *A = nullptr, *B = nullptr, *d = nullptr;  // caller buffers (packed systems)
   int* dinfo = nullptr;
   bool factored = false;
@@ -389,12 +391,18 @@ int upload_tables(bss_plan* p, size_t l) {
 int build(bss_plan* p) {
   if (p->built) return BSS_OK;
   if (int e = layout(p)) return e;
-  hipError_t e = hipMalloc((void**)&p->ws, p->ws_bytes);
-  if (e != hipSuccess) {
-    p->ws = nullptr;
-    return fail(BSS_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", p->ws_bytes, hipGetErrorString(e));
+  if (!p->ws) {
+    hipError_t e = hipMalloc((void**)&p->ws, p->ws_bytes);
+    if (e != hipSuccess) {
+      p->ws = nullptr;
+      return fail(BSS_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", p->ws_bytes, hipGetErrorString(e));
+    }
+    p->ws_owned = true;
   }
-  HIP_TRY(hipMemset(p->ws, 0, p->ws_bytes));
+  // set-up traffic goes through the null stream and is complete before this returns,
+  // so kernels on any (also non-blocking) stream see it
+  HIP_TRY(hipMemsetAsync(p->ws, 0, p->ws_bytes, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
   p->dinfo = (int*)(p->ws + p->info_off);
   if (p->mode == BSS_MODE_MIXED) {
     std::vector<int> nu(p->nsys);
@@ -858,6 +866,7 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
   if (p->mode == BSS_MODE_MIXED && p->mx.bind((const double*)A, (const double*)B, (const double*)d) != 0)
     return fail(BSS_E_HIP, "mixed bind: %s", bss::mixed_last_error());
   if (int e = upload_tables(p, 0)) return e;
+  HIP_TRY(hipStreamSynchronize(nullptr));
   p->factored = false;
   return BSS_OK;
 }
@@ -954,11 +963,25 @@ int bss_sync(bss_plan_t p) {
   return BSS_OK;
 }
 
+int bss_plan_set_workspace(bss_plan_t p, void* ws, size_t bytes) {
+  if (int e = check_plan(p)) return e;
+  if (p->built) return fail(BSS_E_STATE, "the workspace must be set before bind");
+  if (int e = layout(p)) return e;
+  if (!ws) return fail(BSS_E_ARG, "workspace is null");
+  if (((uintptr_t)ws & 255) != 0) return fail(BSS_E_ARG, "workspace must be 256-byte aligned");
+  if (bytes < p->ws_bytes)
+    return fail(BSS_E_ARG, "workspace of %zu bytes is smaller than the %zu the plan needs", bytes,
+                p->ws_bytes);
+  p->ws = (char*)ws;
+  p->ws_owned = false;
+  return BSS_OK;
+}
+
 int bss_plan_destroy(bss_plan_t p) {
   if (!p) return BSS_OK;
   if (p->ws) {
     (void)hipStreamSynchronize(p->last_stream);
-    (void)hipFree(p->ws);
+    if (p->ws_owned) (void)hipFree(p->ws);
   }
   delete p;
   return BSS_OK;

@@ -1,0 +1,309 @@
+"""Request loop: continuous batching of independent block-tridiagonal systems on one
+GPU, admitted by HBM footprint, with factors kept resident for repeated solves.
+
+The reference has no serving layer: its callers factorize and solve one data object at
+a time (benchmark/benchmark_kf.jl:227-331 times one smoother system per call,
+test/test_rocm.jl:95-107 one system per test).  This module is the request loop
+around the same plans:
+
+* ``Request`` -- one system (A, B in pinned host memory, reference block layout) with
+  1..8 right-hand sides to solve against one factorisation, and an arrival time.
+* ``Server.serve`` releases requests at their arrival times into a FIFO.  The head of
+  the FIFO is admitted as soon as (a) a worker is free and (b) its device footprint --
+  plan workspace + A, B, d, as the plan reports it before any allocation -- fits in the
+  HBM budget beside the requests in flight.  There is no batch boundary: a request
+  that arrives while others run starts as soon as a slot and memory free up.
+* Each worker thread owns one HIP stream and runs its request end to end: upload,
+  ``factorize``, then for each right-hand side upload, ``solve``, download.  The
+  factors stay resident between the solves.  Work of different workers overlaps on
+  the device (ctypes releases the GIL during library calls).  Plans take their
+  workspace from torch's caching allocator, so admission and release make no device
+  allocation or free.
+* Latency of a request = host time its last solution is available - arrival time.
+
+A request's arithmetic depends only on its own (N, n, type, mode) plan, never on what
+else is in flight, so a served answer equals the answer of the same request solved
+alone (bitwise for plain plans).
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .solver import _dtype_code, factorize, initialize_batched, solve
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of serve()
+    A: np.ndarray           # (n, n, N) column-major blocks
+    B: np.ndarray           # (n, n, N-1)
+    rhs: List[np.ndarray]   # each (n, nrhs, N)
+
+    @property
+    def N(self) -> int:
+        return self.A.shape[2]
+
+    @property
+    def n(self) -> int:
+        return self.A.shape[0]
+
+    @property
+    def nrhs(self) -> int:
+        return self.rhs[0].shape[1]
+
+
+@dataclass
+class Result:
+    rid: int
+    x: List[np.ndarray]                 # one (n, nrhs, N) solution per right-hand side
+    arrival: float
+    admitted: float
+    finished: float
+    solve_ms: List[float] = field(default_factory=list)  # per right-hand side, device
+    info: int = 0
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.finished - self.arrival) * 1e3
+
+
+def _storage_order(blocks: np.ndarray) -> torch.Tensor:
+    """(r, c, i) column-major blocks -> pinned (i, c, r) C-order tensor, which is the
+    plan tensors' storage order (one contiguous DMA per upload)."""
+    return torch.from_numpy(np.ascontiguousarray(np.transpose(blocks, (2, 1, 0)))).pin_memory()
+
+
+@functools.lru_cache(maxsize=4096)
+def _workspace_bytes(N: int, n: int, nrhs: int, dcode: int, mcode: int) -> int:
+    p = ctypes.c_void_p()
+    L.call("bss_plan_create", N, n, nrhs, 1, dcode, mcode, L.BSS_SCHED_AUTO, ctypes.byref(p))
+    try:
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_WORKSPACE_BYTES, ctypes.byref(v))
+        return int(v.value)
+    finally:
+        L.lib().bss_plan_destroy(p)
+
+
+def request_footprint(N: int, n: int, nrhs: int, dtype=torch.float64, mode: str = "plain") -> int:
+    """Device bytes a request holds while in flight: the plan's workspace (queried from a
+    host-only plan, nothing allocated) plus its A, B and d tensors."""
+    mcode = L.BSS_MODE_MIXED if mode == "mixed" else L.BSS_MODE_PLAIN
+    ws = _workspace_bytes(N, n, nrhs, _dtype_code(dtype), mcode)
+    es = 8 if dtype == torch.float64 else 4
+    return ws + 256 + es * n * (2 * N * n + N * nrhs)
+
+
+class _Staged:
+    """A request's host data in pinned memory, prepared before it is served."""
+
+    def __init__(self, req: Request, dtype):
+        npt = np.float64 if dtype == torch.float64 else np.float32
+        self.A = _storage_order(req.A.astype(npt))
+        self.B = _storage_order(req.B.astype(npt))
+        self.d = [_storage_order(d.astype(npt)) for d in req.rhs]
+        self.out = [torch.empty_like(d).pin_memory() for d in self.d]
+
+
+class Server:
+    """Continuous-batching request loop over libbss plans on one device.
+
+    ``mode``/``dtype`` apply to every request; ``hbm_budget`` (bytes) bounds the summed
+    footprint of the requests in flight (default: 80% of the device's free memory);
+    ``workers`` is the number of requests in flight at most (one HIP stream each)."""
+
+    def __init__(self, mode: str = "plain", dtype=torch.float64, device="cuda",
+                 hbm_budget: Optional[int] = None, workers: int = 8, **plan_options):
+        if not torch.cuda.is_available():
+            raise RuntimeError("Server needs a HIP device; none is visible")
+        self.mode = mode
+        self.dtype = dtype
+        self.device = torch.device(device)
+        self.workers = workers
+        self.plan_options = plan_options
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.stats = {"max_inflight_bytes": 0, "max_inflight": 0}
+
+    # -- admission ---------------------------------------------------------------------
+    def footprint(self, N: int, n: int, nrhs: int) -> int:
+        return request_footprint(N, n, nrhs, self.dtype, self.mode)
+
+    # -- one request on one worker stream ----------------------------------------------
+    def _run(self, req: Request, st: _Staged, stream: torch.cuda.Stream, t0: float,
+             admitted: float) -> Result:
+        N = req.N
+        with torch.cuda.stream(stream):
+            data = initialize_batched(N, req.n, self.dtype, self.device, nrhs=req.nrhs,
+                                      mode=self.mode, torch_workspace=True,
+                                      **self.plan_options)
+            try:
+                data._a[0].copy_(st.A, non_blocking=True)
+                if N > 1:
+                    data._b[0].copy_(st.B, non_blocking=True)
+                factorize(data)
+                xs, solve_ms = [], []
+                for d, out in zip(st.d, st.out):
+                    data._d[0].copy_(d, non_blocking=True)
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    solve(data)
+                    e1.record(stream)
+                    out.copy_(data._d[0], non_blocking=True)
+                    stream.synchronize()
+                    solve_ms.append(e0.elapsed_time(e1))
+                    xs.append(np.transpose(out.numpy(), (2, 1, 0)).copy())
+                finished = time.perf_counter() - t0
+                info = int(data.info()[0])
+            finally:
+                data.close()
+        return Result(req.rid, xs, req.arrival, admitted, finished, solve_ms, info)
+
+    # -- the loop ----------------------------------------------------------------------
+    def serve(self, requests: List[Request], realtime: bool = True) -> List[Result]:
+        """Serve `requests` (arrival times relative to the call) and return one Result
+        per request, in request order.  With ``realtime=False`` every request is
+        available at once (a closed-loop throughput run)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            fp = self.footprint(r.N, r.n, r.nrhs)
+            if fp > self.hbm_budget:
+                raise ValueError(f"request {r.rid} needs {fp} bytes, more than the HBM "
+                                 f"budget of {self.hbm_budget}")
+        staged = {r.rid: _Staged(r, self.dtype) for r in reqs}
+        streams = [torch.cuda.Stream(self.device) for _ in range(self.workers)]
+        free_streams = list(range(self.workers))
+        done: "queue.Queue[Tuple[int, int, object]]" = queue.Queue()
+        results: Dict[int, Result] = {}
+        used = 0
+        inflight: Dict[int, int] = {}   # rid -> footprint
+        fifo: List[Request] = []
+        nxt = 0
+        t0 = time.perf_counter()
+
+        def worker(r: Request, si: int, admitted: float):
+            try:
+                res = self._run(r, staged[r.rid], streams[si], t0, admitted)
+                done.put((r.rid, si, res))
+            except BaseException as exc:  # reported to the loop, re-raised there
+                done.put((r.rid, si, exc))
+
+        threads = []
+        while len(results) < len(reqs):
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and (not realtime or reqs[nxt].arrival <= now):
+                fifo.append(reqs[nxt])
+                nxt += 1
+            # admit in arrival order while a worker and memory are free
+            while fifo and free_streams:
+                r = fifo[0]
+                fp = self.footprint(r.N, r.n, r.nrhs)
+                if used + fp > self.hbm_budget:
+                    break
+                fifo.pop(0)
+                used += fp
+                inflight[r.rid] = fp
+                self.stats["max_inflight_bytes"] = max(self.stats["max_inflight_bytes"], used)
+                self.stats["max_inflight"] = max(self.stats["max_inflight"], len(inflight))
+                si = free_streams.pop()
+                th = threading.Thread(target=worker, args=(r, si, time.perf_counter() - t0),
+                                      daemon=True)
+                th.start()
+                threads.append(th)
+            # completions free their worker and memory
+            timeout = 0.0005
+            if nxt < len(reqs) and realtime:
+                timeout = max(0.0, min(timeout, reqs[nxt].arrival - now))
+            try:
+                rid, si, res = done.get(timeout=timeout if (fifo or nxt < len(reqs)) else 1.0)
+            except queue.Empty:
+                continue
+            while True:
+                if isinstance(res, BaseException):
+                    for th in threads:
+                        th.join()
+                    raise res
+                results[rid] = res
+                used -= inflight.pop(rid)
+                free_streams.append(si)
+                try:
+                    rid, si, res = done.get_nowait()
+                except queue.Empty:
+                    break
+        for th in threads:
+            th.join()
+        return [results[r.rid] for r in requests]
+
+
+def summarize(results: List[Result]) -> dict:
+    """p50/p99 request latency, throughput and solve-latency percentiles."""
+    lat = np.array([r.latency_ms for r in results])
+    solves = np.array([t for r in results for t in r.solve_ms])
+    span = max(r.finished for r in results) - min(r.arrival for r in results)
+    return {
+        "requests": len(results),
+        "systems_per_s": len(results) / span if span > 0 else float("inf"),
+        "solves_per_s": len(solves) / span if span > 0 else float("inf"),
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "solve_ms_p50": float(np.percentile(solves, 50)),
+        "solve_ms_p99": float(np.percentile(solves, 99)),
+        "queue_ms_p50": float(np.percentile([(r.admitted - r.arrival) * 1e3 for r in results], 50)),
+    }
+
+
+def smoother_requests(count: int, rate: float, n_choices=(16, 32, 64), N_range=(128, 4096),
+                      rhs_range=(1, 8), nrhs: int = 1, seed: int = 0) -> List[Request]:
+    """Served-stream workload: Poisson arrivals at `rate` per second of state-space
+    smoother systems (workloads.smoother_system), state size n uniform over
+    `n_choices`, horizon N ~ U[N_range], 1..8 right-hand sides per factorisation."""
+    from .workloads import smoother_system
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(rng.choice(n_choices))
+        N = int(rng.integers(N_range[0], N_range[1] + 1))
+        reps = int(rng.integers(rhs_range[0], rhs_range[1] + 1))
+        sysm = smoother_system(N, n, nrhs=nrhs, seed=int(rng.integers(1 << 31)))
+        rhs = [np.asfortranarray(rng.uniform(size=(n, nrhs, N))) for _ in range(reps)]
+        out.append(Request(k, t, sysm.A, sysm.B, rhs))
+    return out
+
+
+def solve_alone(req: Request, mode: str = "plain", dtype=torch.float64, device="cuda",
+                **plan_options) -> List[np.ndarray]:
+    """The same request on its own plan, on the default stream, nothing else in
+    flight: the reference point for served answers."""
+    data = initialize_batched(req.N, req.n, dtype, device, nrhs=req.nrhs, mode=mode,
+                              **plan_options)
+    try:
+        npt = np.float64 if dtype == torch.float64 else np.float32
+        data.A_tensor.copy_(torch.from_numpy(req.A.astype(npt)))
+        if req.N > 1:
+            data.B_tensor.copy_(torch.from_numpy(req.B.astype(npt)))
+        factorize(data)
+        xs = []
+        for d in req.rhs:
+            data.d_tensor.copy_(torch.from_numpy(d.astype(npt)))
+            solve(data)
+            torch.cuda.synchronize()
+            xs.append(data.d_tensor.cpu().numpy().copy())
+        return xs
+    finally:
+        data.close()

@@ -68,6 +68,21 @@ def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, pres
         L.call("bss_set_option", plan, L.BSS_OPT_PIVOT_TOL, float(pivot_tol))
 
 
+def _torch_workspace(plan, device) -> torch.Tensor:
+    """Workspace for the plan from torch's caching allocator (bss_plan_set_workspace):
+    no hipMalloc/hipFree per plan, the memory returns to the allocator's pool of the
+    current stream when the data object is closed."""
+    v = ctypes.c_int64()
+    L.call("bss_plan_get", plan, L.BSS_Q_WORKSPACE_BYTES, ctypes.byref(v))
+    ws = torch.empty(int(v.value) + 256, dtype=torch.uint8, device=device)
+    off = (-ws.data_ptr()) % 256
+    L.call("bss_plan_set_workspace", plan, ws.data_ptr() + off, int(v.value))
+    # bind initialises the workspace outside the stream order: a block recycled by the
+    # allocator must have no pending work left
+    torch.cuda.current_stream(device).synchronize()
+    return ws
+
+
 class BlockTriDiagData_batched:
     """Counterpart of ``BlockTriDiagData_batched`` (src/gpu_batched.jl:1-71): the
     user-visible tensors plus an opaque libbss plan holding the schedule and the
@@ -79,7 +94,7 @@ class BlockTriDiagData_batched:
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
                  factor_waves: Optional[int] = None, graph: bool = False,
-                 pivot_tol: Optional[float] = None):
+                 pivot_tol: Optional[float] = None, torch_workspace: bool = False):
         self.n_user = N
         self.n = n
         self.nrhs = nrhs
@@ -119,6 +134,7 @@ class BlockTriDiagData_batched:
         if NU > N:
             eye = torch.eye(n, dtype=self.dtype, device=self.device)
             a[:, N - 1:] = eye
+        self._ws = _torch_workspace(plan, self.device) if torch_workspace else None
         with torch.cuda.device(self.device):
             L.call("bss_plan_bind", plan, a.data_ptr(), b.data_ptr() if b.numel() else None,
                    dd.data_ptr())
@@ -178,6 +194,7 @@ class BlockTriDiagData_batched:
         if getattr(self, "_plan", None) is not None and self._plan.value:
             L.lib().bss_plan_destroy(self._plan)
             self._plan = ctypes.c_void_p()
+        self._ws = None
 
     def __del__(self):
         try:
@@ -202,7 +219,7 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
                  factor_waves: Optional[int] = None, graph: bool = False,
-                 pivot_tol: Optional[float] = None):
+                 pivot_tol: Optional[float] = None, torch_workspace: bool = False):
         Ns = [int(v) for v in Ns]
         self.Ns_user = Ns
         self.n = n
@@ -243,6 +260,7 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
                 a[oa + N - 1:oa + NU] = torch.eye(n, dtype=self.dtype, device=self.device)
             oa += NU
             ob += nb
+        self._ws = _torch_workspace(plan, self.device) if torch_workspace else None
         with torch.cuda.device(self.device):
             L.call("bss_plan_bind", plan, a.data_ptr(), b.data_ptr(), dd.data_ptr())
         self.levels = self._get(L.BSS_Q_LEVELS)
@@ -259,7 +277,8 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
                        chain_len_levels: Optional[List[int]] = None,
                        factor_waves: Optional[int] = None,
                        tuned: bool = True, graph: bool = False,
-                       pivot_tol: Optional[float] = None) -> BlockTriDiagData_batched:
+                       pivot_tol: Optional[float] = None,
+                       torch_workspace: bool = False) -> BlockTriDiagData_batched:
     """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
 
     ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
@@ -273,7 +292,8 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     otherwise (``tuned=True``) it comes from the autotuner cache when one exists;
     ``graph=True`` replays each launch sequence from a captured hipGraph;
     ``pivot_tol`` (mixed) is the FP32 separator pivot threshold U_jj^2 / A_jj below
-    which the separator levels are re-factored in FP64."""
+    which the separator levels are re-factored in FP64; ``torch_workspace`` takes the
+    plan's workspace from torch's caching allocator instead of a hipMalloc."""
     if not torch.cuda.is_available():
         raise RuntimeError("initialize_batched needs a HIP device; none is visible")
     device = torch.device(M) if not isinstance(M, torch.device) else M
@@ -290,7 +310,8 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
             factor_waves = cfg.get("factor_waves")
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
                                     base_max, preserve_input, tol, max_iters,
-                                    chain_len_levels, factor_waves, graph, pivot_tol)
+                                    chain_len_levels, factor_waves, graph, pivot_tol,
+                                    torch_workspace)
 
 
 def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *,
@@ -301,7 +322,8 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
                              chain_len_levels: Optional[List[int]] = None,
                              factor_waves: Optional[int] = None,
                              tuned: bool = True, graph: bool = False,
-                             pivot_tol: Optional[float] = None) -> BlockTriDiagData_group:
+                             pivot_tol: Optional[float] = None,
+                             torch_workspace: bool = False) -> BlockTriDiagData_group:
     """Uneven batch: like ``initialize_batched`` but system s has ``Ns[s]`` blocks.
     Schedule options apply to every system.  With ``tuned=True`` the autotuner cache is
     used only when it names one configuration for all the sizes in ``Ns``, so each
@@ -323,7 +345,7 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
             factor_waves = cfgs[0].get("factor_waves")
     return BlockTriDiagData_group(list(Ns), n, T, device, nrhs, mode, schedule, chain_len,
                                   base_max, preserve_input, tol, max_iters, chain_len_levels,
-                                  factor_waves, graph, pivot_tol)
+                                  factor_waves, graph, pivot_tol, torch_workspace)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:

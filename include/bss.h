@@ -138,6 +138,15 @@ int bss_set_option(bss_plan_t plan, int option, double value);
 /* Query plan properties (see enum bss_query). */
 int bss_plan_get(bss_plan_t plan, int query, int64_t* value);
 
+/* Give the plan caller-owned device memory for its workspace instead of the
+ * hipMalloc that bss_plan_bind would make: at least BSS_Q_WORKSPACE_BYTES bytes,
+ * 256-byte aligned, valid until bss_plan_destroy (which then frees nothing), with no
+ * device work pending on it when bss_plan_bind initialises it.  Call before
+ * bss_plan_bind.  Lets a caller that recycles memory (a request loop on a
+ * caching allocator) avoid a device allocation and free per plan.  No counterpart in
+ * the reference (create_data allocates per level, src/gpu_batched.jl:73-110). */
+int bss_plan_set_workspace(bss_plan_t plan, void* ws, size_t bytes);
+
 /* Bind the caller's device buffers (reference tensors A_tensor, B_tensor, d_tensor
  * of BlockTriDiagData_batched, src/gpu_batched.jl:14-27).  A and B are read by
  * bss_factorize; in plain mode they are overwritten with the factors (as the

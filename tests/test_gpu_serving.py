@@ -1,0 +1,55 @@
+"""Request loop on the GPU: requests that arrive while others are in flight are
+admitted by HBM footprint and overlap on the device; every served answer equals the
+same request solved alone (bitwise for plain FP64, within the refinement tolerance for
+mixed) and meets the oracle's backward-error bar."""
+import numpy as np
+import pytest
+import torch
+
+from blockstructuredsolvers_amd import serving
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(count, rate, seed):
+    return serving.smoother_requests(count, rate, n_choices=(16, 32, 64), N_range=(40, 700),
+                                     rhs_range=(1, 4), seed=seed)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_served_equals_alone(mode):
+    reqs = small_stream(14, rate=2000.0, seed=11)
+    srv = serving.Server(mode=mode, workers=4, tuned=False)
+    res = srv.serve(reqs)
+    assert srv.stats["max_inflight"] >= 2  # requests overlapped
+    summ = serving.summarize(res)
+    assert summ["latency_ms_p99"] >= summ["latency_ms_p50"] > 0
+    for r, out in zip(reqs, res):
+        assert out.rid == r.rid and out.info == 0 and len(out.x) == len(r.rhs)
+        alone = serving.solve_alone(r, mode=mode, tuned=False)
+        for k, (xs, xa) in enumerate(zip(out.x, alone)):
+            if mode == "plain":
+                assert np.array_equal(xs, xa), (r.rid, k)
+            else:
+                assert np.abs(xs - xa).max() <= 1e-12 * np.abs(xa).max(), (r.rid, k)
+            berr = oracle.backward_error(r.A, r.B, xs, r.rhs[k]).max()
+            assert berr <= (1e-13 if mode == "plain" else 1e-12), (r.rid, k, berr)
+
+
+def test_admission_respects_hbm_budget():
+    """With a budget that fits only the two largest requests, the loop never holds
+    more than the budget in flight and still serves everything."""
+    reqs = small_stream(10, rate=0.0, seed=5)  # all arrive at once
+    fps = sorted(serving.request_footprint(r.N, r.n, r.nrhs) for r in reqs)
+    budget = fps[-1] + fps[-2]
+    srv = serving.Server(workers=8, hbm_budget=budget, tuned=False)
+    res = srv.serve(reqs, realtime=False)
+    assert srv.stats["max_inflight_bytes"] <= budget
+    assert len(res) == len(reqs)
+    for r, out in zip(reqs, res):
+        xo, info = oracle.solve_system(r.A, r.B, r.rhs[0])
+        assert info == 0
+        assert np.abs(out.x[0] - xo).max() <= 1e-12 * np.abs(xo).max()
+    with pytest.raises(ValueError):
+        serving.Server(hbm_budget=fps[0] - 1).serve(reqs[:1])

@@ -170,3 +170,23 @@ def test_group_argument_errors_and_options():
         assert L.lib().bss_set_option(p, L.BSS_OPT_CHAIN_LEN, 2.0) == -4
     finally:
         L.lib().bss_plan_destroy(p)
+
+
+def test_serving_workload_and_footprint():
+    """Served-stream generator (Poisson arrivals, sizes in range, smoother blocks) and the
+    admission footprint (host-only plan query, grows with N and n)."""
+    from blockstructuredsolvers_amd import serving
+    reqs = serving.smoother_requests(60, rate=100.0, n_choices=(16, 32), N_range=(20, 60),
+                                     rhs_range=(1, 8), seed=3)
+    t = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(t) > 0)
+    assert 0.3 < t[-1] / (60 / 100.0) < 3.0  # mean inter-arrival ~ 1/rate
+    for r in reqs:
+        assert r.n in (16, 32) and 20 <= r.N <= 60 and 1 <= len(r.rhs) <= 8
+        assert r.A.shape == (r.n, r.n, r.N) and r.B.shape == (r.n, r.n, r.N - 1)
+        assert np.allclose(r.A[:, :, 0], r.A[:, :, 0].T)
+        # smoother blocks are SPD: A_i = Q^-1 + F^T Q^-1 F + H^T H
+        assert np.linalg.eigvalsh(r.A[:, :, 0]).min() > 0
+    f = serving.request_footprint
+    assert f(100, 32, 1) < f(200, 32, 1) < f(200, 64, 1)
+    assert f(100, 32, 1, mode="mixed") > f(100, 32, 1)
