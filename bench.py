@@ -82,7 +82,8 @@ def run_path(mode, systems, args, torch, bss):
                                   schedule=args.schedule, chain_len=args.chain_len,
                                   base_max=args.base_max, preserve_input=(mode == "plain"),
                                   tol=args.tol if mode == "mixed" else None,
-                                  chain_len_levels=args.chain_lens, factor_waves=args.factor_waves)
+                                  chain_len_levels=args.chain_lens, factor_waves=args.factor_waves,
+                                  graph=bool(args.graph))
     upload(data, systems, torch)
     d0 = data._d.clone()
     torch.cuda.synchronize()
@@ -182,6 +183,76 @@ def roofline_for(res, args, nsys):
                 launches=count)
 
 
+def factor_gflops(res, args):
+    """Algorithmic factor rate: (7/3) N n^3 flops per system over the summed time of the
+    factorisation kernels (all levels, reduced-diagonal assembly included)."""
+    kt = res["kernels"]
+    steps_ms = sum(t for (k, _), (c, t) in kt.items() if k in ("factor", "assemble_diag"))
+    flops = 7.0 / 3.0 * args.N * args.n ** 3 * args.nsys * args.steps
+    return flops / (steps_ms * 1e-3) / 1e9 if steps_ms > 0 else None
+
+
+def kernel_rooflines(res, args, nsys):
+    """Top-level kernels of one path against the HBM roof (algorithmic bytes)."""
+    out = {}
+    for (name, level) in sorted(res["kernels"]):
+        if level != 0 or name not in ("factor", "forward", "backward", "residual"):
+            continue
+        sub = dict(res)
+        sub["kernels"] = {(name, level): res["kernels"][(name, level)]}
+        r = roofline_for(sub, args, nsys)
+        out[name] = {"per_launch_ms": r["per_launch_ms"], "achieved": r["achieved"],
+                     "unit": r["unit"], "frac": r["frac"]}
+    return out
+
+
+def alpha_point(alpha, args, torch, bss, oracle):
+    """Mixed path on one system with A_i = G G^T + alpha n I: iterations to the backward-
+    error target and the error against the FP64 oracle.  The literal generator with
+    B ~ 0.5 N(0,1) is not SPD below alpha ~ 1e-1 at n = 64, so the SPD-by-construction
+    variant (block-bidiagonal R^T R + alpha n I) is used at every alpha."""
+    from blockstructuredsolvers_amd import workloads
+    sysm = workloads.spd_bidiag_system(args.N, args.n, nrhs=args.nrhs, seed=args.seed, alpha=alpha)
+    data = bss.initialize_batched(args.N, args.n, torch.float64, "cuda", nrhs=args.nrhs,
+                                  mode="mixed", tol=args.tol, max_iters=30)
+    upload(data, [sysm], torch)
+    d0 = data._d.clone()
+    ts = []
+    for k in range(4):
+        data._d.copy_(d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bss.factorize(data)
+        bss.solve(data)
+        torch.cuda.synchronize()
+        if k:
+            ts.append(time.perf_counter() - t0)
+    x = data.d_tensor.cpu().numpy()
+    out = {"alpha": alpha, "iterations": data.iterations,
+           "refactored_levels": data.refactored,
+           "device_berr": float(data.backward_error().max()),
+           "systems_per_s": 1.0 / float(np.median(ts))}
+    data.close()
+    if not args.no_oracle:
+        xo, info = oracle.solve_system(sysm.A, sysm.B, sysm.d)
+        out["oracle_berr"] = float(oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max())
+        out["rel_diff_vs_oracle"] = float(np.abs(x - xo).max() / np.abs(xo).max())
+    return out
+
+
+def load_traffic(kernel, mode, args):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy of known size) for this workload, if recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    key = f"{kernel}|{mode}|N{args.N}|n{args.n}|nrhs{args.nrhs}|nsys{args.nsys}"
+    entry = table.get(key)
+    return entry["bytes"] if entry else None
+
+
 def cpu_baseline(system, seconds, threads):
     """The FP64 oracle (C restatement of the reference's cpu.jl path) solving copies of
     one system, one system per thread, until `seconds` of wall time have passed."""
@@ -225,6 +296,7 @@ def main():
     ap.add_argument("--chain-lens", type=lambda v: [int(x) for x in v.split(",")], default=None,
                     help="per-level chain lengths, top level first")
     ap.add_argument("--factor-waves", type=int, default=None)
+    ap.add_argument("--graph", type=int, default=0, help="1: replay hipGraph-captured levels")
     ap.add_argument("--lat-solves", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -232,6 +304,10 @@ def main():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--modes", default="plain,mixed")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--alphas", default="1,1e-2,1e-4",
+                    help="conditioning sweep of the mixed path (empty: skip)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -278,9 +354,29 @@ def main():
                          f"nrhs={args.nrhs} system in {el:.1f} s, one system per thread "
                          "(C restatement of cpu.jl, oracle/bss_oracle.c)"}
 
+    if args.calib_copy:
+        # a plain copy of known size: HBM counters of the kernels above are scaled by
+        # what FETCH_SIZE / WRITE_SIZE read for it
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    sweep = []
+    if args.alphas and "mixed" in args.modes.split(","):
+        for a in [float(v) for v in args.alphas.split(",") if v]:
+            log(f"bench: conditioning sweep alpha={a:g}")
+            sweep.append(alpha_point(a, args, torch, bss, oracle))
+            log(f"    {sweep[-1]}")
+
     head = results.get("mixed") or results.get("plain")
     plain = results.get("plain")
     roof = roofline_for(head, args, args.nsys)
+    traffic = load_traffic(roof["kernel"], head["mode"], args)
+    if traffic is not None:
+        roof["traffic"] = traffic
     out = {
         "metric": "systems/s, mixed vs plain FP64 at <=1e-12 backward err; p50/p99 solve ms",
         "value": head["systems_per_s"],
@@ -306,6 +402,9 @@ def main():
         "solve_ms_p50": {m: r["solve_ms_p50"] for m, r in results.items()},
         "solve_ms_p99": {m: r["solve_ms_p99"] for m, r in results.items()},
         "accuracy_vs_fp64_oracle": acc,
+        "factor_gflops": {m: factor_gflops(r, args) for m, r in results.items()},
+        "kernel_rooflines": {m: kernel_rooflines(r, args, args.nsys) for m, r in results.items()},
+        "conditioning_sweep": sweep,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

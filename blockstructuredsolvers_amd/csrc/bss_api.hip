@@ -697,8 +697,8 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
   if (!Ns) return fail(BSS_E_ARG, "Ns is null");
   for (int s = 0; s < nsys; ++s)
     if (Ns[s] < 1) return fail(BSS_E_ARG, "N must be >= 1 (system %d has %d)", s, Ns[s]);
-  if (n != 16 && n != 32 && n != 48 && n != 64)
-    return fail(BSS_E_UNSUPPORTED, "block size n=%d not supported (16, 32, 48, 64)", n);
+  if (n != 16 && n != 32 && n != 48 && n != 64 && n != 96 && n != 128)
+    return fail(BSS_E_UNSUPPORTED, "block size n=%d not supported (16, 32, 48, 64, 96, 128)", n);
   if (nrhs < 1 || nrhs > 16) return fail(BSS_E_UNSUPPORTED, "nrhs=%d not in [1, 16]", nrhs);
   if (dtype != BSS_F32 && dtype != BSS_F64) return fail(BSS_E_ARG, "bad dtype %d", dtype);
   if (mode != BSS_MODE_PLAIN && mode != BSS_MODE_MIXED) return fail(BSS_E_ARG, "bad mode %d", mode);

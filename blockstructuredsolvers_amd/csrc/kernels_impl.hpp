@@ -3,6 +3,7 @@
 #pragma once
 
 #include "launch.hpp"
+#include "chain_factor_big.hpp"
 #include "level_kernels.hpp"
 
 namespace bss {
@@ -10,6 +11,12 @@ namespace bss {
 template <typename Tin, typename T, int NB, int NW>
 static hipError_t factor_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
   hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename Tin, typename T, int NB, int NW>
+static hipError_t factor_big_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
+  hipLaunchKernelGGL((chain_factor_big_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -23,6 +30,8 @@ hipError_t launch_factor(int nb, int waves, const FactorLevelArgs& a, int nchain
     case 64:
       if (waves == 8) return factor_nb<Tin, T, 64, 8>(a, nchains, s);
       return factor_nb<Tin, T, 64, 4>(a, nchains, s);
+    case 96: return factor_big_nb<Tin, T, 96, 8>(a, nchains, s);
+    case 128: return factor_big_nb<Tin, T, 128, 8>(a, nchains, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -35,7 +44,7 @@ hipError_t launch_forward(int nb, const SolveLevelArgs& a, int nchains, hipStrea
   case NB:                                                                                   \
     hipLaunchKernelGGL((chain_forward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a);  \
     return hipGetLastError();
-    BSS_FWD(16) BSS_FWD(32) BSS_FWD(48) BSS_FWD(64)
+    BSS_FWD(16) BSS_FWD(32) BSS_FWD(48) BSS_FWD(64) BSS_FWD(96) BSS_FWD(128)
 #undef BSS_FWD
     default: return hipErrorInvalidValue;
   }
@@ -49,7 +58,7 @@ hipError_t launch_backward(int nb, const SolveLevelArgs& a, int nchains, hipStre
   case NB:                                                                                    \
     hipLaunchKernelGGL((chain_backward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a);  \
     return hipGetLastError();
-    BSS_BWD(16) BSS_BWD(32) BSS_BWD(48) BSS_BWD(64)
+    BSS_BWD(16) BSS_BWD(32) BSS_BWD(48) BSS_BWD(64) BSS_BWD(96) BSS_BWD(128)
 #undef BSS_BWD
     default: return hipErrorInvalidValue;
   }

@@ -30,6 +30,9 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // products already hold, so ||M||_inf costs no extra pass over A and B.
 // One wave per 16-row tile; a rolling three-block window of x_new lives in LDS and
 // each block's A/B operands are issued before the window update to hide HBM latency.
+// Every coupling block is read from HBM once: at block row i, B_i serves both
+// B_i x_{i+1} (row i) and B_i^T x_i (row i+1, carried to the next iteration in
+// registers); the second read of B_i follows the first immediately and hits L2.
 template <int NB, bool INIT>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
@@ -71,19 +74,57 @@ __global__ void __launch_bounds__(4 * NB)
       if (j >= b0 && j < b1) xn[(long long)j * ND + e] = v;
     }
   };
+  // k index of operand slot q: 4 consecutive k per lane (contiguous loads along k)
+  auto kof = [&](int q) { return (q >> 2) * 16 + 4 * lk + (q & 3); };
 
-  for (int blk = b0; blk < b1; ++blk) {
-    // operands of this block row, issued first
-    double aA[KQ], aBt[KQ], aB[KQ];
-    if (!INIT) {
+  // carry: B_{i-1}^T x_{i-1} for the current row (and its |.| row sums for ||M||_inf)
+  acc_t carry = Mfma<double>::zero();
+  double rs_carry = 0.0;
+  __syncthreads();  // window zeroed
+  {
+    double aBt[KQ];
+    const bool has = !INIT && b0 > 0;
+    if (has) {
+#pragma unroll
+      for (int q = 0; q < KQ; ++q)
+        aBt[q] = Bs[(b0 - 1) * NN + (long long)(16 * w + lr) * NB + kof(q)];
+    }
+    stage(b0 - 1);
+    stage(b0);
+    __syncthreads();
+    if (has) {
+      const double* Xm = X[(((b0 - 1) % 3) + 3) % 3];
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
-        const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
-        aA[q] = As[blk * NN + (long long)k * NB + 16 * w + lr];
-        aBt[q] = blk > 0 ? Bs[(blk - 1) * NN + (long long)(16 * w + lr) * NB + k] : 0.0;
-        aB[q] = blk + 1 < N ? Bs[blk * NN + (long long)k * NB + 16 * w + lr] : 0.0;
+        carry = Mfma<double>::mma(aBt[q], Xm[lr * LDV + kof(q)], carry);
+        if (mnorm) rs_carry += fabs(aBt[q]);
       }
     }
+  }
+
+  // the k range in NCH chunks of KC operand slots (one chunk up to n = 64), which bounds
+  // the operand registers at n = 96 and 128
+  constexpr int KC = KQ <= 16 ? KQ : KQ / 2;
+  constexpr int NCH = KQ / KC;
+  static_assert(KC * NCH == KQ && KC % 4 == 0, "operand chunks");
+  for (int blk = b0; blk < b1; ++blk) {
+    // operands of this block row, issued first; B_blk twice (row blk, and transposed
+    // for row blk + 1 when that row is ours)
+    const bool next = blk + 1 < N;
+    const bool carry_on = next && blk + 1 < b1;
+    double aA[KC], aB[KC], aBn[KC];
+    auto load_chunk = [&](int ch) {
+#pragma unroll
+      for (int qq = 0; qq < KC; ++qq) {
+        const int k = kof(ch * KC + qq);
+        aA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
+        aB[qq] = next ? Bs[blk * NN + (long long)k * NB + 16 * w + lr] : 0.0;
+      }
+#pragma unroll
+      for (int qq = 0; qq < KC; ++qq)
+        aBn[qq] = carry_on ? Bs[blk * NN + (long long)(16 * w + lr) * NB + kof(ch * KC + qq)] : 0.0;
+    };
+    if (!INIT) load_chunk(0);
     acc_t acc;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -92,28 +133,35 @@ __global__ void __launch_bounds__(4 * NB)
       dmax = nmax(dmax, fabs(acc[v]));
     }
     __syncthreads();  // previous block's window reads are done
-    if (blk == b0) {
-      stage(blk - 1);
-      stage(blk);
-    }
     stage(blk + 1);
     __syncthreads();
-    const double* Xm = X[(((blk - 1) % 3) + 3) % 3];
     const double* X0 = X[blk % 3];
     const double* Xp = X[(blk + 1) % 3];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       xmax = nmax(xmax, fabs(X0[lr * LDV + 16 * w + Mfma<double>::row(lk, v)]));
     if (!INIT) {
-      double rs = 0.0;
 #pragma unroll
-      for (int q = 0; q < KQ; ++q) {
-        const int k = (q >> 2) * 16 + 4 * lk + (q & 3);
-        acc = Mfma<double>::mma(-aA[q], X0[lr * LDV + k], acc);
-        acc = Mfma<double>::mma(-aBt[q], Xm[lr * LDV + k], acc);
-        acc = Mfma<double>::mma(-aB[q], Xp[lr * LDV + k], acc);
-        if (mnorm) rs += fabs(aA[q]) + fabs(aBt[q]) + fabs(aB[q]);
+      for (int v = 0; v < 4; ++v) acc[v] -= carry[v];
+      acc_t cn = Mfma<double>::zero();
+      double rs = rs_carry, rsn = 0.0;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        if (ch > 0) load_chunk(ch);
+#pragma unroll
+        for (int qq = 0; qq < KC; ++qq) {
+          const int k = kof(ch * KC + qq);
+          acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
+          acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
+          cn = Mfma<double>::mma(aBn[qq], X0[lr * LDV + k], cn);
+          if (mnorm) {
+            rs += fabs(aA[qq]) + fabs(aB[qq]);
+            rsn += fabs(aBn[qq]);
+          }
+        }
       }
+      carry = cn;
+      rs_carry = rsn;
       if (mnorm) {
         // the four lanes sharing lr hold the four k-slices of row 16w + lr
         rs += __shfl_xor(rs, 16);
@@ -226,6 +274,8 @@ hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, bool ini
       case 32: e = launch_residual<32>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
       case 48: e = launch_residual<48>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
       case 64: e = launch_residual<64>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      case 96: e = launch_residual<96>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
+      case 128: e = launch_residual<128>(m, init, xo_k, dx_k, xn_k, ro_k, want_mnorm, s); break;
       default: e = hipErrorInvalidValue;
     }
     if (scope) h.prof_end(scope);

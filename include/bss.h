@@ -110,7 +110,7 @@ enum bss_kernel_kind {
  * right-hand-side columns.  Replaces initialize_batched (src/gpu_batched.jl:165-182):
  * the schedule search and the per-level workspace allocation happen here (device
  * memory is allocated on the current HIP device); the caller owns A, B, d.
- * Supported n: 16, 32, 48, 64 (FP32 and FP64), 1 <= nrhs <= 16. */
+ * Supported n: 16, 32, 48, 64, 96, 128 (FP32 and FP64), 1 <= nrhs <= 16. */
 int bss_plan_create(int N, int n, int nrhs, int nsys, int dtype, int mode, int schedule,
                     bss_plan_t* out);
 

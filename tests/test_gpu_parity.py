@@ -42,7 +42,7 @@ def check_f64(sysm, x):
     return rel, berr
 
 
-@pytest.mark.parametrize("n", [16, 32, 48, 64])
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128])
 @pytest.mark.parametrize("schedule", ["auto", "reference"])
 def test_f64_small(n, schedule):
     sysm = workloads.spd_system(50, n, nrhs=1, seed=7 + n)
@@ -85,13 +85,29 @@ def test_f64_chain_lengths(chain_len):
     check_f64(sysm, x)
 
 
-def test_f64_multi_rhs16():
-    sysm = workloads.spd_system(200, 64, nrhs=16, seed=5)
+@pytest.mark.parametrize("n", [64, 128])
+def test_f64_multi_rhs16(n):
+    sysm = workloads.spd_system(200, n, nrhs=16, seed=5)
     x, _ = run_gpu(sysm)
     check_f64(sysm, x)
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [96, 128])
+def test_big_blocks_multilevel_and_info(n):
+    """n = 96 / 128 (triangle-in-LDS factor kernel): several levels, chain lengths
+    1..4, a non-SPD block reported at its index."""
+    sysm = workloads.spd_system(700, n, nrhs=3, seed=40 + n)
+    for cl in (1, 3, 4):
+        x, data = run_gpu(sysm, chain_len=cl, base_max=2, tuned=False)
+        assert data.levels >= 3
+        check_f64(sysm, x)
+    A = sysm.A.copy(order="F")
+    A[:, :, 333] = -np.eye(n)
+    _, data = run_gpu(workloads.BlockSystem(A, sysm.B, sysm.d), tuned=False)
+    assert data.info()[0] > 0
+
+
+@pytest.mark.parametrize("n", [32, 64, 96, 128])
 def test_f32_matches_reference_tolerance(n):
     sysm = workloads.spd_system(50, n, nrhs=1, seed=21, manufactured=True)
     s32 = workloads.BlockSystem(sysm.A.astype(np.float32, order="F"),
@@ -143,7 +159,7 @@ def run_gpu_batch(systems, T=torch.float64, **kw):
     return [data.d_tensor[:, :, :N, s].double().cpu().numpy() for s in range(len(systems))], data
 
 
-@pytest.mark.parametrize("n,nrhs", [(32, 1), (64, 1), (64, 16), (16, 3)])
+@pytest.mark.parametrize("n,nrhs", [(32, 1), (64, 1), (64, 16), (16, 3), (96, 2), (128, 1), (128, 16)])
 def test_mixed_refined_backward_error(n, nrhs):
     """FP32 MFMA factorisation + FP64 refinement: normwise backward error <= 1e-12 as
     measured by the FP64 oracle, agreeing with the device's own measurement."""

@@ -69,7 +69,7 @@ def _chain_factors(A, B):
     return Uinv, C
 
 
-@pytest.mark.parametrize("n", [16, 32, 48, 64])
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128])
 def test_single_chain_factors(n):
     """N <= base size: the whole system is one chain; after factorize (plain FP64) the A
     slots hold Uinv_i (upper 16x16 tiles) and the B slots C_i."""
