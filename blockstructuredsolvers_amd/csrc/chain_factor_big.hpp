@@ -10,8 +10,9 @@
 // sides B and Z and the products C, W live in their HBM output slots: the triangular
 // solve reads them tile by tile (L2-resident, just written), holds its output tiles in
 // registers across a barrier and writes them back in place, and the Schur products
-// read C and W from there.  Global data handed from one wave to another passes a
-// device-scope fence (the vector L1 may hold a line a wave read before it changed).
+// read C and W from there.  Global data handed from one wave to another needs only the
+// workgroup barrier: its workgroup-scope release/acquire orders the stores and the
+// workgroup's waves share one CU's vector L1.
 #pragma once
 
 #include "chain_factor.hpp"
@@ -62,11 +63,9 @@ __device__ inline void tile_mma_k(typename Mfma<T>::acc_t& acc, const T* __restr
                                  TB ? yb + (long long)k0 * ldy : yb + k0, ldy);
 }
 
-// global writes of this workgroup visible to all of its waves, then barrier
-__device__ inline void global_handoff() {
-  __threadfence();
-  __syncthreads();
-}
+// global writes of this workgroup visible to all of its waves: the barrier's
+// workgroup-scope fences (a device-scope fence would write back the whole L2)
+__device__ inline void global_handoff() { __syncthreads(); }
 
 template <typename Tin, typename T, int NB, int NW>
 __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelArgs p) {
@@ -127,7 +126,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
       upper_tile(s, a, b);
       acc_t acc = tile_load_conv<Tin, T>(A + blk * NN + 16 * b * NB + 16 * a, NB);
       if (i > 0)
-        tile_mma_k<T, true, false, true, NB>(acc, C + (blk - 1) * NN + 16 * a * NB, NB,
+        tile_mma<T, true, false, true, NB>(acc, C + (blk - 1) * NN + 16 * a * NB, NB,
                                            C + (blk - 1) * NN + 16 * b * NB, NB);
       tile_store<T>(tri + Tri::idx(a, b), LDT, acc);
     }
@@ -216,6 +215,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         const int c = task / TT, r = (task % TT + c) % TT;
         X[q] = Mfma<T>::zero();
         if (task < TT * TT) {
+#pragma unroll 2
           for (int t = 0; t <= r; ++t)
             tile_mma<T, true, false, false, 16>(X[q], tri + Tri::idx(t, r), LDT,
                                                 base + 16 * c * NB + 16 * t, NB);
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         if (s < NS) {
           int a, b;
           upper_tile(s, a, b);
-          tile_mma_k<T, true, false, false, NB>(Sacc[q], Wb + 16 * a * NB, NB, Wb + 16 * b * NB, NB);
+          tile_mma<T, true, false, false, NB>(Sacc[q], Wb + 16 * a * NB, NB, Wb + 16 * b * NB, NB);
         }
       }
     }
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         for (int task = w; task < TT * TT; task += NW) {
           const int a = task % TT, b = task / TT;
           acc_t acc = Mfma<T>::zero();
-          tile_mma_k<T, true, false, true, NB>(acc, Cb + 16 * a * NB, NB, Wb + 16 * b * NB, NB);
+          tile_mma<T, true, false, true, NB>(acc, Cb + 16 * a * NB, NB, Wb + 16 * b * NB, NB);
           tile_store<T>(z + 16 * b * NB + 16 * a, NB, acc);
         }
       }
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         int a, b;
         upper_tile(s, a, b);
         acc_t acc = Mfma<T>::zero();
-        tile_mma_k<T, true, false, false, NB>(acc, Cb + 16 * a * NB, NB, Cb + 16 * b * NB, NB);
+        tile_mma<T, true, false, false, NB>(acc, Cb + 16 * a * NB, NB, Cb + 16 * b * NB, NB);
         tile_store<T>(sr + 16 * b * NB + 16 * a, NB, acc);
       }
       if (hasL) {
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         for (int task = w; task < TT * TT; task += NW) {
           const int a = task % TT, b = task / TT;
           acc_t acc = Mfma<T>::zero();
-          tile_mma_k<T, true, false, true, NB>(acc, Wb + 16 * a * NB, NB, Cb + 16 * b * NB, NB);
+          tile_mma<T, true, false, true, NB>(acc, Wb + 16 * a * NB, NB, Cb + 16 * b * NB, NB);
           tile_store<T>(bn + 16 * b * NB + 16 * a, NB, acc);
         }
       }
