@@ -233,8 +233,18 @@ struct BlockStage {
   }
 };
 
+// FP32 at n = 64 with 8 waves: LDS (70 KB) leaves room for two workgroups per CU, so
+// registers are capped at 128 (4 waves per SIMD) to let two chains overlap their
+// serial phases; measured faster than 191 registers and one workgroup per CU.
+template <typename T, int NB, int NW>
+constexpr int factor_min_waves_per_eu() {
+  return (sizeof(T) == 4 && NB == 64 && NW == 8) ? 4 : 1;
+}
+
 template <typename Tin, typename T, int NB, int NW>
-__global__ void __launch_bounds__(64 * NW) chain_factor_kernel(FactorLevelArgs p) {
+__global__ void __launch_bounds__(64 * NW)
+    __attribute__((amdgpu_waves_per_eu(factor_min_waves_per_eu<T, NB, NW>())))
+    chain_factor_kernel(FactorLevelArgs p) {
   constexpr int TT = NB / 16;
   constexpr int LD = NB + LdsPad<T>::value;
   constexpr int NT = 64 * NW;

@@ -64,6 +64,42 @@ __device__ inline void load_vec(const T* g, T* v, int nrhs) {
   for (int i = threadIdx.x; i < NB * nrhs; i += NT) v[(i / NB) * LDV + (i % NB)] = g[i];
 }
 
+// The HBM operand of a 16-row product, loaded ahead into registers: op(X) rows
+// i = 0..15 over the 16-deep k slices [s_lo, s_hi) (TA: X[k][i] at xb[i*ldx + k], else
+// X[i][k] at xb[k*ldx + i]), in tile_mma's k order.
+template <typename T, int NB, bool TA>
+struct XOps {
+  T a[NB / 16][4];
+  __device__ inline void load(const T* __restrict__ xb, int ldx, int s_lo = 0,
+                              int s_hi = NB / 16) {
+    const int l = lane_id();
+    const int lr = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int s = 0; s < NB / 16; ++s)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = 16 * s + 4 * lk + q;
+        a[s][q] = (s >= s_lo && s < s_hi) ? (TA ? xb[lr * ldx + k] : xb[k * ldx + lr]) : T(0);
+      }
+  }
+  // acc (+/-)= op(X) Y over the slices [s_lo, s_hi), Y[k][j] at yb[j*ldy + k] (LDS)
+  template <bool NEG>
+  __device__ inline void mma(typename Mfma<T>::acc_t& acc, const T* __restrict__ yb, int ldy,
+                             int s_lo = 0, int s_hi = NB / 16) const {
+    const int l = lane_id();
+    const int lr = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int s = 0; s < NB / 16; ++s) {
+      if (s < s_lo || s >= s_hi) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const T b = yb[lr * ldy + 16 * s + 4 * lk + q];
+        acc = Mfma<T>::mma(NEG ? -a[s][q] : a[s][q], b, acc);
+      }
+    }
+  }
+};
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p) {
   constexpr int TT = NB / 16;
@@ -90,23 +126,31 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   zero_vec<T, LDV, NT>(R);
   __syncthreads();
 
+  // Each HBM operand panel of block i+1 is loaded into the registers of block i's panel
+  // once that panel's product has been consumed by a later instruction (so the
+  // products have retired before their source registers are overwritten); the load
+  // latency then hides behind the rest of block i.
+  XOps<T, NB, true> oc, ou, ow;  // C_{i-1} panel, Uinv column tiles t <= w, W_i panel
+  ou.load(U + (long long)start * NN + 16 * w * NB, NB, 0, w + 1);
+  if (sepL >= 0) ow.load(W + (long long)start * NN + 16 * w * NB, NB);
   acc_t accL = Mfma<T>::zero();
   for (int i = 0; i < len; ++i) {
     const long long blk = start + i;
     acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
-    if (i > 0)
-      tile_mma<T, true, false, true, NB>(acc, C + (blk - 1) * NN + 16 * w * NB, NB, Y, LDV);
+    if (i > 0) oc.template mma<true>(acc, Y, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
+    if (i + 1 < len) oc.load(C + blk * NN + 16 * w * NB, NB);  // C_blk, used by row blk + 1
+    // W_blk, into registers whose products (row blk - 1) retired before the C product
+    // that tile_store above waited on
+    if (sepL >= 0 && i > 0) ow.load(W + blk * NN + 16 * w * NB, NB);
     __syncthreads();
     acc = Mfma<T>::zero();
-    for (int t = 0; t <= w; ++t)
-      tile_mma<T, true, false, false, 16>(acc, U + blk * NN + 16 * w * NB + 16 * t, NB, R + 16 * t,
-                                      LDV);
+    ou.template mma<false>(acc, R, LDV, 0, w + 1);  // Uinv_blk^T r
     store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
     tile_store<T>(Y + 16 * w, LDV, acc);
+    if (i + 1 < len) ou.load(U + (blk + 1) * NN + 16 * w * NB, NB, 0, w + 1);
     __syncthreads();
-    if (sepL >= 0)
-      tile_mma<T, true, false, false, NB>(accL, W + blk * NN + 16 * w * NB, NB, Y, LDV);
+    if (sepL >= 0) ow.template mma<false>(accL, Y, LDV);
   }
   if (sepR >= 0) {
     const long long last = start + len - 1;
@@ -149,20 +193,32 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
   if (sepL >= 0) load_vec<T, NB, LDV, NT>(xs + sepL * ND, XL, nrhs);
   __syncthreads();
 
+  // operand panels of block i-1 are loaded right after block i consumed them
+  XOps<T, NB, false> oc, ow, ou;  // rows 16w.. of C_i, W_i, and of Uinv_i (tiles t >= w)
+  {
+    const long long blk = start + len - 1;
+    // C of the last block exists only when a right separator follows it (a level's
+    // last block has no coupling block)
+    if (sepR >= 0) oc.load(C + blk * NN + 16 * w, NB);
+    if (sepL >= 0) ow.load(W + blk * NN + 16 * w, NB);
+    ou.load(U + blk * NN + 16 * w, NB, w, TT);
+  }
   for (int i = len - 1; i >= 0; --i) {
     const long long blk = start + i;
     acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
-    if (i < len - 1 || sepR >= 0)
-      tile_mma<T, false, false, true, NB>(acc, C + blk * NN + 16 * w, NB, Xn, LDV);
-    if (sepL >= 0) tile_mma<T, false, false, true, NB>(acc, W + blk * NN + 16 * w, NB, XL, LDV);
-    tile_store<T>(R + 16 * w, LDV, acc);
+    if (i < len - 1 || sepR >= 0) oc.template mma<true>(acc, Xn, LDV);
+    if (sepL >= 0) ow.template mma<true>(acc, XL, LDV);
+    tile_store<T>(R + 16 * w, LDV, acc);  // consumes the products above
+    if (i > 0) {
+      oc.load(C + (blk - 1) * NN + 16 * w, NB);
+      if (sepL >= 0) ow.load(W + (blk - 1) * NN + 16 * w, NB);
+    }
     __syncthreads();
     acc = Mfma<T>::zero();
-    for (int t = w; t < TT; ++t)
-      tile_mma<T, false, false, false, 16>(acc, U + blk * NN + 16 * t * NB + 16 * w, NB, R + 16 * t,
-                                       LDV);
+    ou.template mma<false>(acc, R, LDV, w, TT);  // Uinv_blk t
     store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
     tile_store<T>(Xn + 16 * w, LDV, acc);
+    if (i > 0) ou.load(U + (blk - 1) * NN + 16 * w, NB, w, TT);
     __syncthreads();
   }
 }

@@ -102,9 +102,9 @@ __global__ void __launch_bounds__(4 * NB)
     }
   }
 
-  // the k range in NCH chunks of KC operand slots (one chunk up to n = 64), which bounds
-  // the operand registers at n = 96 and 128
-  constexpr int KC = KQ <= 16 ? KQ : KQ / 2;
+  // the k range in NCH chunks of KC operand slots (two chunks from n = 64 on), which
+  // bounds the operand registers and measured faster at n = 64
+  constexpr int KC = (KQ <= 8 || KQ % 8 != 0) ? KQ : KQ / 2;
   constexpr int NCH = KQ / KC;
   static_assert(KC * NCH == KQ && KC % 4 == 0, "operand chunks");
   for (int blk = b0; blk < b1; ++blk) {
@@ -244,8 +244,8 @@ __global__ void __launch_bounds__(kRedThreads)
   berr[sys * nrhs + c] = den > 0.0 ? r / den : r;
 }
 
-// blocks per residual workgroup: at least 8, and at most 1024 workgroups per system
-int blocks_per_item(int N) { return std::max(8, (N + 1023) / 1024); }
+// blocks per residual workgroup: at least 16, and at most 512 workgroups per system
+int blocks_per_item(int N) { return std::max(16, (N + 511) / 512); }
 
 template <int NB>
 hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, int xn_k,
