@@ -215,7 +215,6 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         const int c = task / TT, r = (task % TT + c) % TT;
         X[q] = Mfma<T>::zero();
         if (task < TT * TT) {
-#pragma unroll 2
           for (int t = 0; t <= r; ++t)
             tile_mma<T, true, false, false, 16>(X[q], tri + Tri::idx(t, r), LDT,
                                                 base + 16 * c * NB + 16 * t, NB);
