@@ -49,6 +49,7 @@ SIGNATURES = {
     "bss_plan_get": ([_VP, _I, ctypes.POINTER(ctypes.c_int64)], _I),
     "bss_plan_set_workspace": ([_VP, _VP, ctypes.c_size_t], _I),
     "bss_plan_bind": ([_VP, _VP, _VP, _VP], _I),
+    "bss_build_smoother": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP], _I),
     "bss_factorize": ([_VP, _VP], _I),
     "bss_solve": ([_VP, _VP], _I),
     "bss_info": ([_VP, ctypes.POINTER(_I)], _I),

@@ -722,6 +722,10 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
 
 }  // namespace
 
+namespace bss {
+int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace bss
+
 extern "C" {
 
 int bss_last_error(char* buf, size_t len) {

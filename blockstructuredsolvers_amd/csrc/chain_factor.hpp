@@ -91,21 +91,30 @@ __device__ inline float rsqrt_nr(float p) {
 // 16-lane row; the four rows compute the same thing) holds column c: a[r] = A[r][c],
 // e[r] = E[r][c].  Everything the step needs is in the lane's own registers or in
 // lane J of the same row, so the serial path uses only readlane/DPP and VALU ops.
+// `s` = 1/sqrt of pivot J.  Row J+1 is updated first: it holds the next pivot, whose
+// 1/sqrt (the long dependent part of a step) then overlaps this step's other rows.
 template <typename T, int J>
-__device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&e)[16], bool& ok) {
-  // keep steps apart in the schedule: interleaving them buys no latency (each step
-  // depends on the previous pivot) and multiplies the live registers
+__device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&e)[16], T s,
+                                                                bool& ok) {
+  // keep steps apart in the schedule: interleaving whole steps multiplies the live
+  // registers (the pivot of step J+1 is already computed inside step J)
   __builtin_amdgcn_sched_barrier(0);
-  const T p = read_lane(a[J], J);  // pivot A[J][J]
-  ok = ok && (p > T(0));
-  const T s = rsqrt_nr(p);
   const T ujc = a[J] * s;  // U[J][c]
   const T ejc = e[J] * s;  // row J of E, scaled
   a[J] = ujc;
   e[J] = ejc;
-#pragma unroll
-  for (int r = J + 1; r < 16; ++r) {
+  T s_next = T(0);
+  if constexpr (J < 15) {
     // U[J][r] = A[J][r] s = A[r][J] s (the trailing matrix stays symmetric): lane J's a[r]
+    const T uj1 = row_bcast<J>(a[J + 1]) * s;
+    a[J + 1] -= uj1 * ujc;
+    e[J + 1] -= uj1 * ejc;
+    const T p = read_lane(a[J + 1], J + 1);  // pivot A[J+1][J+1]
+    ok = ok && (p > T(0));
+    s_next = rsqrt_nr(p);
+  }
+#pragma unroll
+  for (int r = J + 2; r < 16; ++r) {
     const T ujr = row_bcast<J>(a[r]) * s;
     a[r] -= ujr * ujc;
     e[r] -= ujr * ejc;
@@ -114,7 +123,7 @@ __device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&
   // its updates to the end and keeps every multiplier of every step live
 #pragma unroll
   for (int r = 0; r < 16; ++r) asm volatile("" : "+v"(e[r]));
-  if constexpr (J < 15) diag_step<T, J + 1>(a, e, ok);
+  if constexpr (J < 15) diag_step<T, J + 1>(a, e, s_next, ok);
 }
 
 // Wave-level Cholesky + inverse of one 16x16 SPD tile held (upper triangle) in LDS.
@@ -130,8 +139,9 @@ __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* a
     a[r] = (r <= c) ? at[c * lda + r] : at[r * lda + c];
     e[r] = (r == c) ? T(1) : T(0);
   }
-  bool ok = true;
-  diag_step<T, 0>(a, e, ok);
+  const T p0 = read_lane(a[0], 0);
+  bool ok = p0 > T(0);
+  diag_step<T, 0>(a, e, rsqrt_nr(p0), ok);
   if (lane_id() < 16) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) ut[r * ldu + c] = e[r];  // Uinv[c][r] = E[r][c]

@@ -1,7 +1,9 @@
 """Served-stream benchmark: Poisson arrivals of state-space smoother systems (state size
 n in {16, 32, 64}, horizon N ~ U[128, 4096], 1..8 right-hand sides per factorisation)
-through the request loop, at a sweep of arrival rates.  Prints one JSON line per rate
-with p50/p99 request latency, p50/p99 solve latency and throughput.
+through the request loop, at a sweep of arrival rates.  Requests carry the model and
+the device builds their blocks (``--upload-blocks`` sends dense blocks instead).
+Prints one JSON line per rate with p50/p99 request latency, p50/p99 solve latency and
+throughput.
 
     python -m blockstructuredsolvers_amd.serve_bench --rates 50,200,800 --count 64
 """
@@ -17,8 +19,10 @@ import torch
 from . import serving
 
 
-def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int) -> dict:
-    reqs = serving.smoother_requests(count, rate, N_range=(128, N_max), seed=seed)
+def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
+        device_blocks: bool = True) -> dict:
+    reqs = serving.smoother_requests(count, rate, N_range=(128, N_max), seed=seed,
+                                     device_blocks=device_blocks)
     srv = serving.Server(mode=mode, workers=workers)
     srv.serve(reqs[:min(4, count)], realtime=False)  # warm-up: kernels loaded, pools primed
     t0 = time.perf_counter()
@@ -26,6 +30,7 @@ def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int)
     wall = time.perf_counter() - t0
     out = serving.summarize(res)
     out.update(rate_per_s=rate, mode=mode, workers=workers, wall_s=wall,
+               blocks="device-built from the model" if device_blocks else "uploaded",
                offered_span_s=reqs[-1].arrival, max_inflight=srv.stats["max_inflight"],
                max_inflight_gb=srv.stats["max_inflight_bytes"] / 1e9,
                rhs_total=sum(len(r.rhs) for r in reqs), failed=sum(r.info != 0 for r in res))
@@ -40,12 +45,15 @@ def main(argv=None) -> int:
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--N-max", type=int, default=4096)
+    ap.add_argument("--upload-blocks", action="store_true",
+                    help="send dense A, B with each request instead of the model")
     a = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("serve_bench needs a GPU", file=sys.stderr)
         return 2
     for r in [float(x) for x in a.rates.split(",")]:
-        print(json.dumps(run(r, a.count, a.mode, a.workers, a.seed, a.N_max)), flush=True)
+        print(json.dumps(run(r, a.count, a.mode, a.workers, a.seed, a.N_max,
+                             not a.upload_blocks)), flush=True)
     return 0
 
 

@@ -39,28 +39,41 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .solver import _dtype_code, factorize, initialize_batched, solve
+from .solver import _dtype_code, build_smoother, factorize, initialize_batched, solve
 
 
 @dataclass
 class Request:
+    """One system and its right-hand sides.  The blocks come either as dense arrays (A,
+    B) or as a state-space model (``model``, a ``workloads.SmootherModel``) whose blocks
+    the device builds (``bss_build_smoother``), so they never cross PCIe."""
     rid: int
-    arrival: float          # seconds after the start of serve()
-    A: np.ndarray           # (n, n, N) column-major blocks
-    B: np.ndarray           # (n, n, N-1)
-    rhs: List[np.ndarray]   # each (n, nrhs, N)
+    arrival: float                      # seconds after the start of serve()
+    rhs: List[np.ndarray]               # each (n, nrhs, N)
+    A: Optional[np.ndarray] = None      # (n, n, N) column-major blocks
+    B: Optional[np.ndarray] = None      # (n, n, N-1)
+    model: Optional[object] = None
 
     @property
     def N(self) -> int:
-        return self.A.shape[2]
+        return self.rhs[0].shape[2]
 
     @property
     def n(self) -> int:
-        return self.A.shape[0]
+        return self.rhs[0].shape[0]
 
     @property
     def nrhs(self) -> int:
         return self.rhs[0].shape[1]
+
+    def dense(self):
+        """(A, B) as host arrays (from the model when the request carries one)."""
+        if self.A is not None:
+            return self.A, self.B
+        Ai, Bi = self.model.blocks()
+        A = np.asfortranarray(np.broadcast_to(Ai[:, :, None], (self.n, self.n, self.N)))
+        B = np.asfortranarray(np.broadcast_to(Bi[:, :, None], (self.n, self.n, max(self.N - 1, 0))))
+        return A, B
 
 
 @dataclass
@@ -106,12 +119,15 @@ def request_footprint(N: int, n: int, nrhs: int, dtype=torch.float64, mode: str 
 
 
 class _Staged:
-    """A request's host data in pinned memory, prepared before it is served."""
+    """A request's host data in pinned memory, prepared before it is served (no A, B
+    when the device builds the blocks from the request's model)."""
 
     def __init__(self, req: Request, dtype):
         npt = np.float64 if dtype == torch.float64 else np.float32
-        self.A = _storage_order(req.A.astype(npt))
-        self.B = _storage_order(req.B.astype(npt))
+        self.A = self.B = None
+        if req.model is None:
+            self.A = _storage_order(req.A.astype(npt))
+            self.B = _storage_order(req.B.astype(npt))
         self.d = [_storage_order(d.astype(npt)) for d in req.rhs]
         self.out = [torch.empty_like(d).pin_memory() for d in self.d]
 
@@ -151,9 +167,12 @@ class Server:
                                       mode=self.mode, torch_workspace=True,
                                       **self.plan_options)
             try:
-                data._a[0].copy_(st.A, non_blocking=True)
-                if N > 1:
-                    data._b[0].copy_(st.B, non_blocking=True)
+                if req.model is not None:
+                    build_smoother(data, req.model)
+                else:
+                    data._a[0].copy_(st.A, non_blocking=True)
+                    if N > 1:
+                        data._b[0].copy_(st.B, non_blocking=True)
                 factorize(data)
                 xs, solve_ms = [], []
                 for d, out in zip(st.d, st.out):
@@ -267,11 +286,14 @@ def summarize(results: List[Result]) -> dict:
 
 
 def smoother_requests(count: int, rate: float, n_choices=(16, 32, 64), N_range=(128, 4096),
-                      rhs_range=(1, 8), nrhs: int = 1, seed: int = 0) -> List[Request]:
+                      rhs_range=(1, 8), nrhs: int = 1, seed: int = 0,
+                      device_blocks: bool = False) -> List[Request]:
     """Served-stream workload: Poisson arrivals at `rate` per second of state-space
-    smoother systems (workloads.smoother_system), state size n uniform over
-    `n_choices`, horizon N ~ U[N_range], 1..8 right-hand sides per factorisation."""
-    from .workloads import smoother_system
+    smoother systems (workloads.smoother_model), state size n uniform over
+    `n_choices`, horizon N ~ U[N_range], 1..8 right-hand sides per factorisation.
+    With ``device_blocks`` a request carries the model and the device builds its
+    blocks; otherwise it carries the dense blocks."""
+    from .workloads import smoother_model
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -280,9 +302,13 @@ def smoother_requests(count: int, rate: float, n_choices=(16, 32, 64), N_range=(
         n = int(rng.choice(n_choices))
         N = int(rng.integers(N_range[0], N_range[1] + 1))
         reps = int(rng.integers(rhs_range[0], rhs_range[1] + 1))
-        sysm = smoother_system(N, n, nrhs=nrhs, seed=int(rng.integers(1 << 31)))
+        model = smoother_model(n, np.random.default_rng(int(rng.integers(1 << 31))))
         rhs = [np.asfortranarray(rng.uniform(size=(n, nrhs, N))) for _ in range(reps)]
-        out.append(Request(k, t, sysm.A, sysm.B, rhs))
+        req = Request(k, t, rhs, model=model)
+        if not device_blocks:
+            req.A, req.B = req.dense()
+            req.model = None
+        out.append(req)
     return out
 
 
@@ -294,9 +320,12 @@ def solve_alone(req: Request, mode: str = "plain", dtype=torch.float64, device="
                               **plan_options)
     try:
         npt = np.float64 if dtype == torch.float64 else np.float32
-        data.A_tensor.copy_(torch.from_numpy(req.A.astype(npt)))
-        if req.N > 1:
-            data.B_tensor.copy_(torch.from_numpy(req.B.astype(npt)))
+        if req.model is not None:
+            build_smoother(data, req.model)
+        else:
+            data.A_tensor.copy_(torch.from_numpy(req.A.astype(npt)))
+            if req.N > 1:
+                data.B_tensor.copy_(torch.from_numpy(req.B.astype(npt)))
         factorize(data)
         xs = []
         for d in req.rhs:

@@ -364,6 +364,28 @@ factorize_ = factorize
 solve_ = solve
 
 
+def build_smoother(data: BlockTriDiagData_batched, model) -> None:
+    """Fill ``data.A_tensor``/``data.B_tensor`` (a single-system plan) with the
+    smoother normal equations of ``model`` (``workloads.SmootherModel``), built on the
+    device from F, H, Q^{-1}, R^{-1} (``bss_build_smoother``), asynchronously on the
+    current stream.  The blocks never cross PCIe."""
+    if data.nsys != 1 or isinstance(data, BlockTriDiagData_group):
+        raise ValueError("build_smoother fills a single-system plan")
+    n = data.n
+    if model.F.shape != (n, n):
+        raise ValueError(f"model state size {model.F.shape} does not match n={n}")
+    dev = data.device
+    # column-major device copies: the row-major storage of M^T is M column by column
+    F, H, Qi, Ri = [torch.from_numpy(np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)).to(dev)
+                    for M in (model.F, model.H, model.Qinv, model.Rinv)]
+    code = L.BSS_F64 if data.dtype == torch.float64 else L.BSS_F32
+    with torch.cuda.device(dev):
+        L.call("bss_build_smoother", data.n_user, n, model.H.shape[0], F.data_ptr(),
+               H.data_ptr(), Qi.data_ptr(), Ri.data_ptr(), data._a.data_ptr(),
+               data._b.data_ptr() if data._b.numel() else None, code, _stream_handle(dev))
+    data._model_tensors = (F, H, Qi, Ri)  # alive until the stream has used them
+
+
 def reference_schedule(N: int):
     """The reference's find_sequence_upper (src/gpu_batched.jl:155-162) as computed by
     libbss: (N_use, [(N_i, m_i, P_i) top level first])."""

@@ -99,6 +99,31 @@ def spd_bidiag_system(N: int, n: int, nrhs: int = 1, seed: int = 42, alpha: floa
                        _blocks_to_F(d).astype(dtype, order="F"))
 
 
+@dataclass
+class SmootherModel:
+    """Linear state-space model x_{i+1} = F x_i + w, y_i = H x_i + v with w ~ N(0, Q),
+    v ~ N(0, R); kept as the inverses the normal equations use."""
+    F: np.ndarray     # (n, n)
+    H: np.ndarray     # (m, n)
+    Qinv: np.ndarray  # (n, n)
+    Rinv: np.ndarray  # (m, m)
+
+    def blocks(self):
+        """(A_i, B_i): A = Q^{-1} + F^T Q^{-1} F + H^T R^{-1} H, B = -Q^{-1} F."""
+        A = self.Qinv + self.F.T @ self.Qinv @ self.F + self.H.T @ self.Rinv @ self.H
+        return A, -self.Qinv @ self.F
+
+
+def smoother_model(n: int, rng: np.random.Generator) -> SmootherModel:
+    """The served-stream model: F = 0.95*Q_orth (orthogonal factor of a Gaussian with
+    sign fix, benchmark/benchmark_kf.jl:80-82), H (n/2) x n N(0,1), Q = 0.1 I, R = I."""
+    Z = rng.standard_normal((n, n))
+    Qo, Rr = np.linalg.qr(Z)
+    Qo = Qo * np.sign(np.diag(Rr))[None, :]
+    H = rng.standard_normal((max(n // 2, 1), n))
+    return SmootherModel(0.95 * Qo, H, 10.0 * np.eye(n), np.eye(H.shape[0]))
+
+
 def smoother_system(N: int, n: int, nrhs: int = 1, seed: int = 42,
                     dtype=np.float64) -> BlockSystem:
     """State-space smoother normal equations (benchmark/benchmark_kf.jl:79-154 as the
@@ -107,14 +132,7 @@ def smoother_system(N: int, n: int, nrhs: int = 1, seed: int = 42,
     benchmark uses the transpose of B_i and special-cases A_1, A_N; this follows the
     config text."""
     rng = np.random.default_rng(seed)
-    Z = rng.standard_normal((n, n))
-    Qo, Rr = np.linalg.qr(Z)
-    Qo = Qo * np.sign(np.diag(Rr))[None, :]
-    Fm = 0.95 * Qo
-    H = rng.standard_normal((max(n // 2, 1), n))
-    qinv = 10.0
-    Ai = qinv * np.eye(n) + qinv * Fm.T @ Fm + H.T @ H
-    Bi = -qinv * Fm
+    Ai, Bi = smoother_model(n, rng).blocks()
     A = np.broadcast_to(Ai, (N, n, n)).copy()
     B = np.broadcast_to(Bi, (max(N - 1, 0), n, n)).copy()
     d = rng.uniform(size=(N, n, nrhs))

@@ -167,6 +167,18 @@ int bss_factorize(bss_plan_t plan, void* stream);
  * iterative refinement and synchronises the stream once per iteration. */
 int bss_solve(bss_plan_t plan, void* stream);
 
+/* Build the block-tridiagonal normal equations of a linear state-space smoother on the
+ * device: for the model x_{i+1} = F x_i + w (w ~ N(0, Q)), y_i = H x_i + v (v ~ N(0, R)),
+ * A_i = Q^{-1} + F^T Q^{-1} F + H^T R^{-1} H (all N blocks) and B_i = -Q^{-1} F (N-1
+ * blocks), written in the reference layout (column-major n x n blocks, contiguous)
+ * into A and B, as dtype, on `stream`.  F (n x n), H (m x n), Qinv (n x n) and Rinv
+ * (m x m) are column-major FP64 device arrays.  The reference builds these blocks on
+ * the host (benchmark/benchmark_kf.jl:79-154, with A_1 and A_N special-cased); here a
+ * request carries only the model and its blocks never cross PCIe. */
+int bss_build_smoother(int N, int n, int m, const double* F, const double* H,
+                       const double* Qinv, const double* Rinv, void* A, void* B, int dtype,
+                       void* stream);
+
 /* Per-system numerical status after the last factorize (synchronises the stream):
  * info[s] = 0 if system s factored, else the smallest 1-based block index whose
  * pivot was non-positive.  The reference passes an `info` buffer to potrf and never

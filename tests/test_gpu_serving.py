@@ -12,14 +12,15 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 
-def small_stream(count, rate, seed):
+def small_stream(count, rate, seed, device_blocks=False):
     return serving.smoother_requests(count, rate, n_choices=(16, 32, 64), N_range=(40, 700),
-                                     rhs_range=(1, 4), seed=seed)
+                                     rhs_range=(1, 4), seed=seed, device_blocks=device_blocks)
 
 
-@pytest.mark.parametrize("mode", ["plain", "mixed"])
-def test_served_equals_alone(mode):
-    reqs = small_stream(14, rate=2000.0, seed=11)
+@pytest.mark.parametrize("mode,device_blocks", [("plain", False), ("mixed", False),
+                                                ("plain", True)])
+def test_served_equals_alone(mode, device_blocks):
+    reqs = small_stream(14, rate=2000.0, seed=11, device_blocks=device_blocks)
     srv = serving.Server(mode=mode, workers=4, tuned=False)
     res = srv.serve(reqs)
     assert srv.stats["max_inflight"] >= 2  # requests overlapped
@@ -33,7 +34,8 @@ def test_served_equals_alone(mode):
                 assert np.array_equal(xs, xa), (r.rid, k)
             else:
                 assert np.abs(xs - xa).max() <= 1e-12 * np.abs(xa).max(), (r.rid, k)
-            berr = oracle.backward_error(r.A, r.B, xs, r.rhs[k]).max()
+            A, B = r.dense()
+            berr = oracle.backward_error(A, B, xs, r.rhs[k]).max()
             assert berr <= (1e-13 if mode == "plain" else 1e-12), (r.rid, k, berr)
 
 
@@ -53,3 +55,33 @@ def test_admission_respects_hbm_budget():
         assert np.abs(out.x[0] - xo).max() <= 1e-12 * np.abs(xo).max()
     with pytest.raises(ValueError):
         serving.Server(hbm_budget=fps[0] - 1).serve(reqs[:1])
+
+
+@pytest.mark.parametrize("n,T", [(16, torch.float64), (64, torch.float64), (128, torch.float32)])
+def test_device_smoother_blocks_match_host(n, T):
+    """bss_build_smoother: the device-built normal-equation blocks equal the host
+    formula (A = Q^-1 + F^T Q^-1 F + H^T R^-1 H, B = -Q^-1 F) in every block."""
+    from blockstructuredsolvers_amd import build_smoother, factorize, initialize_batched, solve
+    from blockstructuredsolvers_amd.workloads import smoother_model
+    N = 37
+    model = smoother_model(n, np.random.default_rng(n))
+    Ai, Bi = model.blocks()
+    data = initialize_batched(N, n, T, "cuda", tuned=False)
+    build_smoother(data, model)
+    torch.cuda.synchronize()
+    A = data.A_tensor.double().cpu().numpy()
+    B = data.B_tensor.double().cpu().numpy()
+    tol = 1e-13 if T == torch.float64 else 1e-6
+    for i in range(N):
+        assert np.abs(A[:, :, i] - Ai).max() <= tol * np.abs(Ai).max(), i
+    for i in range(N - 1):
+        assert np.abs(B[:, :, i] - Bi).max() <= tol * np.abs(Bi).max(), i
+    if T == torch.float64:
+        d = np.asfortranarray(np.random.default_rng(1).uniform(size=(n, 1, N)))
+        data.d_tensor.copy_(torch.from_numpy(d))
+        factorize(data)
+        solve(data)
+        torch.cuda.synchronize()
+        Ah = np.asfortranarray(np.repeat(Ai[:, :, None], N, axis=2))
+        Bh = np.asfortranarray(np.repeat(Bi[:, :, None], N - 1, axis=2))
+        assert oracle.backward_error(Ah, Bh, data.d_tensor.cpu().numpy(), d).max() <= 1e-13

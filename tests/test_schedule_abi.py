@@ -181,6 +181,12 @@ def test_serving_workload_and_footprint():
     t = np.array([r.arrival for r in reqs])
     assert np.all(np.diff(t) > 0)
     assert 0.3 < t[-1] / (60 / 100.0) < 3.0  # mean inter-arrival ~ 1/rate
+    dev = serving.smoother_requests(60, rate=100.0, n_choices=(16, 32), N_range=(20, 60),
+                                    rhs_range=(1, 8), seed=3, device_blocks=True)
+    for r, m in zip(reqs, dev):  # same stream either way; the model expands to the blocks
+        assert m.A is None and m.model is not None and m.N == r.N and m.arrival == r.arrival
+        A, B = m.dense()
+        assert np.array_equal(A, r.A) and np.array_equal(B, r.B)
     for r in reqs:
         assert r.n in (16, 32) and 20 <= r.N <= 60 and 1 <= len(r.rhs) <= 8
         assert r.A.shape == (r.n, r.n, r.N) and r.B.shape == (r.n, r.n, r.N - 1)
