@@ -52,17 +52,6 @@ __device__ inline typename Mfma<T>::acc_t tile_load_conv(const Tin* p, int ld) {
   return a;
 }
 
-// acc += op(X) op(Y) over K = NB in 16-deep steps that are not all unrolled, which
-// keeps the operand registers of a full-depth product bounded at large n
-template <typename T, bool TA, bool TB, bool NEG, int K>
-__device__ inline void tile_mma_k(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
-                                  int ldx, const T* __restrict__ yb, int ldy) {
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 16)
-    tile_mma<T, TA, TB, NEG, 16>(acc, TA ? xb + k0 : xb + (long long)k0 * ldx, ldx,
-                                 TB ? yb + (long long)k0 * ldy : yb + k0, ldy);
-}
-
 // global writes of this workgroup visible to all of its waves: the barrier's
 // workgroup-scope fences (a device-scope fence would write back the whole L2)
 __device__ inline void global_handoff() { __syncthreads(); }

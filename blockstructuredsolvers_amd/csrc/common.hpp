@@ -89,6 +89,17 @@ __device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restric
   }
 }
 
+// acc += op(X) op(Y) over K in 16-deep steps unrolled by two (not fully), which keeps
+// the operand registers of a full-depth product bounded
+template <typename T, bool TA, bool TB, bool NEG, int K>
+__device__ inline void tile_mma_k(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
+                                  int ldx, const T* __restrict__ yb, int ldy) {
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 16)
+    tile_mma<T, TA, TB, NEG, 16>(acc, TA ? xb + k0 : xb + (long long)k0 * ldx, ldx,
+                                 TB ? yb + (long long)k0 * ldy : yb + k0, ldy);
+}
+
 // Load / store a D-layout tile from/to a column-major buffer (element (0,0) at p).
 template <typename T>
 __device__ inline typename Mfma<T>::acc_t tile_load(const T* p, int ld) {
