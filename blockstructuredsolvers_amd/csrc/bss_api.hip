@@ -63,7 +63,7 @@ struct SysLevel {
   bss::LevelShape shape;
   int P = 0;
   size_t oA = kNone, oB = kNone, od = kNone, oW = kNone, oSR = kNone, oSL = kNone,
-         orR = kNone, orL = kNone, oU = kNone, oC = kNone, oOrig = kNone;
+         orR = kNone, orL = kNone, oU = kNone, oC = kNone, oOrig = kNone, oPrev = kNone;
   // mixed mode, levels >= 1: FP64 copies used when the separator levels are
   // re-factored in FP64
   size_t wA = kNone, wB = kNone, wd = kNone, wW = kNone, wSR = kNone, wSL = kNone,
@@ -82,7 +82,8 @@ struct GLevel {
   std::vector<int> slot_sys;  // slot -> system
   bss::ChainRec* dchains = nullptr;
   bss::SepRec* dseps = nullptr;
-  bss::LevelSys* dtab[2] = {nullptr, nullptr};  // [1] differs at level 0 only (d vector)
+  // [k]: level-0 vector set k (mixed mode ping-pong); differs at levels 0 and 1 only
+  bss::LevelSys* dtab[2] = {nullptr, nullptr};
   // mixed mode: level >= 1 -> FP64 tables; level 0 -> hand-off tables whose next-level
   // pointers (An, dn) name the FP64 copies of level 1
   bss::LevelSys* dtab64[2] = {nullptr, nullptr};
@@ -258,6 +259,7 @@ int layout(bss_plan* p) {
         V.orL = bump.take(V.P * nd * fs);
       }
       V.oOrig = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
+      if (l > 0) V.oPrev = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
       if (mixed && l > 0) {
         // the FP32 factor keeps its C apart from B, so the reduced couplings survive
         // for an FP64 re-factorisation
@@ -297,7 +299,7 @@ int layout(bss_plan* p) {
     GLevel& G = p->gl[l];
     p->toff[l].chains = bump.take(sizeof(bss::ChainRec) * std::max<size_t>(G.chains.size(), 1));
     p->toff[l].seps = bump.take(sizeof(bss::SepRec) * std::max<size_t>(G.seps.size(), 1));
-    for (int k = 0; k < (l == 0 ? 2 : 1); ++k) {
+    for (int k = 0; k < (l <= 1 ? 2 : 1); ++k) {
       p->toff[l].tab[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
       if (mixed) p->toff[l].tab64[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
     }
@@ -371,13 +373,24 @@ bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k, bool wide = f
   t.orig = (const int*)at(V.oOrig);
   t.sys = s;
   t.N = V.shape.N;
+  // the sweeps do the level hand-off themselves (not in the FP64 separator tables,
+  // whose top hand-off converts between types in separate kernels)
+  t.fused = wide ? 0 : 1;
+  if (!wide && l > 0) {
+    const SysLevel& Pv = S.lv[l - 1];
+    t.dp = level_sys(p, s, l - 1, k, false).d;
+    t.rRp = at(Pv.orR);
+    t.rLp = at(Pv.orL);
+    t.pp = (const int*)at(V.oPrev);
+  }
   return t;
 }
 
 int upload_tables(bss_plan* p, size_t l) {
   GLevel& G = p->gl[l];
-  for (int k = 0; k < (l == 0 ? 2 : 1); ++k) {
+  for (int k = 0; k < (l <= 1 ? 2 : 1); ++k) {
     for (int wide = 0; wide < (p->mode == BSS_MODE_MIXED ? 2 : 1); ++wide) {
+      if (wide && l == 1 && k == 1) continue;  // FP64 tables vary at level 0 only
       std::vector<bss::LevelSys> tab;
       for (int s : G.slot_sys) tab.push_back(level_sys(p, s, l, k, wide != 0));
       HIP_TRY(hipMemcpy(wide ? G.dtab64[k] : G.dtab[k], tab.data(),
@@ -415,7 +428,7 @@ int build(bss_plan* p) {
     G.dchains = (bss::ChainRec*)(p->ws + p->toff[l].chains);
     G.dseps = (bss::SepRec*)(p->ws + p->toff[l].seps);
     for (int k = 0; k < 2; ++k) {
-      G.dtab[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab[l == 0 ? k : 0]);
+      G.dtab[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab[l <= 1 ? k : 0]);
       if (p->mode == BSS_MODE_MIXED)
         G.dtab64[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab64[l == 0 ? k : 0]);
     }
@@ -438,6 +451,9 @@ int build(bss_plan* p) {
       std::vector<int> next(V.P);
       for (int k = 0; k < V.P; ++k) next[k] = orig[V.shape.seppos[k]];
       orig.swap(next);
+      if (l + 1 < S.lv.size() && V.P > 0)  // next level's block q sits at seppos[q] here
+        HIP_TRY(hipMemcpy(p->ws + S.lv[l + 1].oPrev, V.shape.seppos.data(), V.P * sizeof(int),
+                          hipMemcpyHostToDevice));
     }
   }
   if (p->mode == BSS_MODE_MIXED) {
@@ -563,37 +579,26 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
 template <typename T>
 int solve_levels(bss_plan* p, int k, hipStream_t s) {
   const int nb = p->n;
-  const int nd = p->n * p->nrhs;
   const size_t L = p->gl.size();
   for (size_t l = 0; l < L; ++l) {
     GLevel& G = p->gl[l];
-    const bss::SolveLevelArgs a{G.dchains, G.dtab[l == 0 ? k : 0], p->nrhs};
+    const bss::SolveLevelArgs a{G.dchains, G.dtab[k], p->nrhs};
     hipError_t e;
     {
       ProfScope ps(p, BSS_K_FORWARD, (int)l, s);
       e = bss::launch_forward<T>(nb, a, (int)G.chains.size(), s);
     }
     if (e != hipSuccess) return fail(BSS_E_HIP, "forward level %zu: %s", l, hipGetErrorString(e));
-    if (!G.seps.empty()) {
-      ProfScope ps(p, BSS_K_ASSEMBLE_RHS, (int)l, s);
-      e = bss::launch_assemble_rhs<T, T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
-      if (e != hipSuccess) return fail(BSS_E_HIP, "rhs assemble %zu: %s", l, hipGetErrorString(e));
-    }
   }
   for (size_t li = L; li-- > 0;) {
     GLevel& G = p->gl[li];
-    const bss::SolveLevelArgs a{G.dchains, G.dtab[li == 0 ? k : 0], p->nrhs};
+    const bss::SolveLevelArgs a{G.dchains, G.dtab[k], p->nrhs};
     hipError_t e;
     {
       ProfScope ps(p, BSS_K_BACKWARD, (int)li, s);
       e = bss::launch_backward<T>(nb, a, (int)G.chains.size(), s);
     }
     if (e != hipSuccess) return fail(BSS_E_HIP, "backward level %zu: %s", li, hipGetErrorString(e));
-    if (!G.seps.empty()) {
-      ProfScope ps(p, BSS_K_SCATTER, (int)li, s);
-      e = bss::launch_scatter_sep<T, T>(G.dseps, a.sys, nd, (int)G.seps.size(), s);
-      if (e != hipSuccess) return fail(BSS_E_HIP, "scatter %zu: %s", li, hipGetErrorString(e));
-    }
   }
   return BSS_OK;
 }
@@ -675,10 +680,6 @@ int solve_levels_sep64(bss_plan* p, int k, hipStream_t s) {
   if (e == hipSuccess) {
     ProfScope ps(p, BSS_K_BACKWARD, 0, s);
     e = bss::launch_backward<float>(nb, a0, (int)G0.chains.size(), s);
-  }
-  if (e == hipSuccess) {
-    ProfScope ps(p, BSS_K_SCATTER, 0, s);
-    e = bss::launch_scatter_sep<float, float>(G0.dseps, a0.sys, nd, (int)G0.seps.size(), s);
   }
   if (e != hipSuccess) return fail(BSS_E_HIP, "FP32 back-substitution: %s", hipGetErrorString(e));
   return BSS_OK;
@@ -869,7 +870,9 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
   p->d = d;
   if (p->mode == BSS_MODE_MIXED && p->mx.bind((const double*)A, (const double*)B, (const double*)d) != 0)
     return fail(BSS_E_HIP, "mixed bind: %s", bss::mixed_last_error());
-  if (int e = upload_tables(p, 0)) return e;
+  // levels 0 and 1 name the caller's buffers (level 1 reads level 0's vectors)
+  for (size_t l = 0; l < std::min<size_t>(2, p->gl.size()); ++l)
+    if (int e = upload_tables(p, l)) return e;
   HIP_TRY(hipStreamSynchronize(nullptr));
   p->factored = false;
   return BSS_OK;

@@ -112,7 +112,6 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
 
   const ChainRec cd = p.chains[blockIdx.x];
   const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
-  if (len <= 0) return;
   const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
@@ -121,6 +120,27 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   const T* W = (const T*)S.W;
   T* d = (T*)S.d;
   const int w = wave_id();
+  // right-hand side of block q: below the top level (fused hand-off) it is assembled
+  // from the previous level, d[q] = dp[pp[q]] - rRp[q] - rLp[q]
+  const bool assemble = S.fused && S.dp;
+  auto rhs_tile = [&](long long q) -> acc_t {
+    if (!assemble) return load_rhs_tile<T, NB>(d + q * ND, w, nrhs);
+    acc_t a = load_rhs_tile<T, NB>((const T*)S.dp + (long long)S.pp[q] * ND, w, nrhs);
+    const acc_t r = load_rhs_tile<T, NB>((const T*)S.rRp + q * ND, w, nrhs);
+    const acc_t l = load_rhs_tile<T, NB>((const T*)S.rLp + q * ND, w, nrhs);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] = a[v] - r[v] - l[v];
+    return a;
+  };
+  // the separators this chain owns get their assembled right-hand side, for the next
+  // level: the one on its right, and a separator at block 0 (which has no chain on its
+  // left; the reference layout starts a level with one)
+  if (assemble && sepR >= 0) {
+    const long long q = start + len;
+    store_rhs_tile<T, NB>(d + q * ND, w, nrhs, rhs_tile(q));
+  }
+  if (assemble && sepL >= 0 && start == 1) store_rhs_tile<T, NB>(d, w, nrhs, rhs_tile(0));
+  if (len <= 0) return;
 
   zero_vec<T, LDV, NT>(Y);
   zero_vec<T, LDV, NT>(R);
@@ -136,7 +156,7 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   acc_t accL = Mfma<T>::zero();
   for (int i = 0; i < len; ++i) {
     const long long blk = start + i;
-    acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
+    acc_t acc = rhs_tile(blk);
     if (i > 0) oc.template mma<true>(acc, Y, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
     if (i + 1 < len) oc.load(C + blk * NN + 16 * w * NB, NB);  // C_blk, used by row blk + 1
@@ -174,7 +194,6 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
 
   const ChainRec cd = p.chains[blockIdx.x];
   const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
-  if (len <= 0) return;
   const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
@@ -184,6 +203,14 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
   T* d = (T*)S.d;
   const T* xs = (const T*)S.dn;
   const int w = wave_id();
+  // fused hand-off: the solutions of the separators this chain owns (as in the forward
+  // sweep) into this level's vector, for the level above or the caller at the top
+  if (S.fused && sepR >= 0)
+    store_rhs_tile<T, NB>(d + (long long)(start + len) * ND, w, nrhs,
+                          load_rhs_tile<T, NB>(xs + sepR * ND, w, nrhs));
+  if (S.fused && sepL >= 0 && start == 1)
+    store_rhs_tile<T, NB>(d, w, nrhs, load_rhs_tile<T, NB>(xs + sepL * ND, w, nrhs));
+  if (len <= 0) return;
 
   zero_vec<T, LDV, NT>(Xn);
   zero_vec<T, LDV, NT>(XL);

@@ -42,6 +42,16 @@ struct LevelSys {
   const int* orig;  // level-0 block index of every block (info reporting)
   int sys;          // system index
   int N;            // blocks at this level
+  // Level hand-off fused into the sweeps (`fused` != 0): the forward sweep assembles
+  // this level's right-hand side from the previous level, d[q] = dp[pp[q]] - rRp[q] -
+  // rLp[q] (when dp is set, i.e. below the top level), and every chain writes the
+  // separator on its right (assembled in the forward sweep, solved in the backward).
+  const void* dp;   // previous level's vectors
+  const void* rRp;  // previous level's separator contributions
+  const void* rLp;
+  const int* pp;    // position of each block of this level in the previous level
+  int fused;
+  int pad_;
 };
 
 // One buffer to convert element-wise S -> D (count elements): the reduced couplings

@@ -196,3 +196,34 @@ def test_serving_workload_and_footprint():
     f = serving.request_footprint
     assert f(100, 32, 1) < f(200, 32, 1) < f(200, 64, 1)
     assert f(100, 32, 1, mode="mixed") > f(100, 32, 1)
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("N", [2, 3, 5, 9, 17, 50, 100, 257, 2000, 8192, 10007])
+def test_every_separator_has_one_owner(N, schedule):
+    """The solve sweeps do the level hand-off themselves: the chain whose right
+    separator it is, or for a separator at block 0 the chain right of it, assembles and
+    later writes back each separator.  Every separator of every level must have exactly
+    one such owner, in both schedules."""
+    p = ctypes.c_void_p()
+    L.call("bss_plan_create", N, 32, 1, 1, 1, 0, schedule, ctypes.byref(p))
+    try:
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_LEVELS, ctypes.byref(v))
+        for k in range(v.value):
+            L.call("bss_plan_get", p, L.BSS_Q_LEVEL_N + k, ctypes.byref(v))
+            NL = v.value
+            buf = (ctypes.c_int * (4 * (NL + 2)))()
+            nch = L.call("bss_plan_level_chains", p, k, buf, NL + 2)
+            chains = [tuple(buf[4 * c:4 * c + 4]) for c in range(nch)]
+            seps = {st + ln for (st, ln, hl, hr) in chains if hr}
+            seps |= {st - 1 for (st, ln, hl, hr) in chains if hl}
+            owners = {}
+            for (st, ln, hl, hr) in chains:
+                if hr:
+                    owners[st + ln] = owners.get(st + ln, 0) + 1
+                if hl and st == 1:
+                    owners[0] = owners.get(0, 0) + 1
+            assert set(owners) == seps and all(c == 1 for c in owners.values()), (k, owners)
+    finally:
+        L.lib().bss_plan_destroy(p)
