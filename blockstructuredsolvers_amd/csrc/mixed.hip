@@ -199,46 +199,35 @@ __global__ void __launch_bounds__(4 * NB)
 // `mnorm` first reduces ||M||_inf from the same pass.  One 256-thread workgroup per
 // system: each thread folds every 256th partial record, then wave shuffles and LDS
 // finish the reduction.
-constexpr int kRedThreads = 256;
-constexpr int kSlots = 49;  // 3 x 16 column maxima + the |M| row-sum maximum
+constexpr int kRedThreads = 1024;
 
 __global__ void __launch_bounds__(kRedThreads)
     berr_kernel(const ResSys* __restrict__ systems, const double* __restrict__ part,
                 double* __restrict__ mnorm, double* __restrict__ dnorm,
                 double* __restrict__ berr, int nrhs, int init, int want_mnorm) {
-  __shared__ double red[kRedThreads / 64][kSlots];
+  static_assert(kRedThreads % kParts == 0, "a thread keeps one partial slot");
+  __shared__ double red[kRedThreads];
   const int sys = blockIdx.x, tid = threadIdx.x;
   const int g0 = systems[sys].item0, g1 = systems[sys].item1;
-  double m[kSlots];
-#pragma unroll
-  for (int k = 0; k < kSlots; ++k) m[k] = 0.0;
-  for (int g = g0 + tid; g < g1; g += kRedThreads) {
-    const double* pp = part + (long long)g * kParts;
-#pragma unroll
-    for (int k = 0; k < 48; ++k) m[k] = nmax(m[k], pp[k]);
-    m[48] = nmax(m[48], pp[kMnormSlot]);
-  }
-#pragma unroll
-  for (int k = 0; k < kSlots; ++k) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m[k] = nmax(m[k], __shfl_xor(m[k], off));
-  }
-  if ((tid & 63) == 0) {
-#pragma unroll
-    for (int k = 0; k < kSlots; ++k) red[tid >> 6][k] = m[k];
+  // the system's partials are contiguous (item-major, kParts slots each): thread t
+  // reads every kRedThreads-th value, all of slot t % kParts, coalesced
+  const double* base = part + (long long)g0 * kParts;
+  const long long total = (long long)(g1 - g0) * kParts;
+  double m = 0.0;
+  for (long long i = tid; i < total; i += kRedThreads) m = nmax(m, base[i]);
+  red[tid] = m;
+  __syncthreads();
+  if (tid < kParts) {
+    for (int k = 1; k < kRedThreads / kParts; ++k) m = nmax(m, red[tid + k * kParts]);
+    red[tid] = m;
   }
   __syncthreads();
   if (tid >= 16) return;
-  auto fold = [&](int k) {
-    double v = 0.0;
-    for (int w = 0; w < kRedThreads / 64; ++w) v = nmax(v, red[w][k]);
-    return v;
-  };
-  const double mn = want_mnorm ? fold(48) : mnorm[sys];
+  const double mn = want_mnorm ? red[kMnormSlot] : mnorm[sys];
   if (want_mnorm && tid == 0) mnorm[sys] = mn;
   const int c = tid;
   if (c >= nrhs) return;
-  const double r = fold(c), x = fold(16 + c), dd = fold(32 + c);
+  const double r = red[c], x = red[16 + c], dd = red[32 + c];
   if (init) dnorm[sys * 16 + c] = dd;
   const double den = mn * x + (init ? dd : dnorm[sys * 16 + c]);
   berr[sys * nrhs + c] = den > 0.0 ? r / den : r;
