@@ -33,6 +33,8 @@ BSS_OPT_BASE_MAX = 4
 BSS_OPT_PRESERVE_INPUT = 5
 BSS_OPT_FACTOR_WAVES = 6
 BSS_OPT_GRAPH = 7
+BSS_OPT_PROFILE_KINDS = 8
+BSS_OPT_PROFILE_MAX_LEVEL = 9
 BSS_OPT_CHAIN_LEN_LEVEL = 32
 
 KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,

@@ -124,6 +124,8 @@ struct bss_plan {
     hipEvent_t a, b;
   };
   bool prof = false;
+  unsigned prof_kinds = ~0u;
+  int prof_max_level = 1 << 30;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
   size_t pool_next = 0;
@@ -190,7 +192,7 @@ struct ProfScope {
   hipStream_t s;
   long idx = -1;
   ProfScope(bss_plan* p_, int kind, int level, hipStream_t s_) : p(p_), s(s_) {
-    if (!p->prof) return;
+    if (!p->prof || !((p->prof_kinds >> kind) & 1u) || level > p->prof_max_level) return;
     hipEvent_t a = prof_event(p), b = prof_event(p);
     if (!a || !b || hipEventRecord(a, s) != hipSuccess) return;
     p->recs.push_back({kind, level, a, b});
@@ -798,6 +800,15 @@ int bss_set_option(bss_plan_t p, int option, double value) {
       return BSS_OK;
     case BSS_OPT_PRESERVE_INPUT:
       p->preserve = value != 0.0;
+      return BSS_OK;
+    case BSS_OPT_PROFILE_KINDS:
+      if (value < 0 || value >= (double)(1u << BSS_K_COUNT) || value != (double)(unsigned)value)
+        return fail(BSS_E_ARG, "profile kinds: a bit mask below 2^%d", BSS_K_COUNT);
+      p->prof_kinds = (unsigned)value;
+      return BSS_OK;
+    case BSS_OPT_PROFILE_MAX_LEVEL:
+      if (value < 0) return fail(BSS_E_ARG, "profile max level must be >= 0");
+      p->prof_max_level = value > (double)(1 << 30) ? 1 << 30 : (int)value;
       return BSS_OK;
     default:
       if (option >= BSS_OPT_CHAIN_LEN_LEVEL && option < BSS_OPT_CHAIN_LEN_LEVEL + 16) {

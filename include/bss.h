@@ -88,6 +88,12 @@ enum bss_option {
                                mixed mode each FP32 correction solve) into a hipGraph on
                                first use and replay it afterwards (default 0).  Ignored
                                while the profiler is on. */
+  BSS_OPT_PROFILE_KINDS = 8, /* profiler: bit k set = bracket launches of kernel kind k
+                               (enum bss_kernel_kind; default all kinds).  Every
+                               bracketed launch costs an event pair on the stream */
+  BSS_OPT_PROFILE_MAX_LEVEL = 9, /* profiler: bracket launches at levels <= value only
+                               (default: every level; the residual and backward-error
+                               kernels count as level 0) */
   BSS_OPT_CHAIN_LEN_LEVEL = 32 /* + level k (< 16): chain length of level k only
                                   (auto schedule); overrides BSS_OPT_CHAIN_LEN there */
 };

@@ -163,6 +163,12 @@ def test_group_argument_errors_and_options():
         L.call("bss_set_option", p, L.BSS_OPT_GRAPH, 1.0)
         L.call("bss_set_option", p, L.BSS_OPT_PIVOT_TOL, 2.0)
         assert L.lib().bss_set_option(p, L.BSS_OPT_PIVOT_TOL, -1.0) == -1
+        # profiler selection: a mask over the kernel kinds and a top level
+        L.call("bss_set_option", p, L.BSS_OPT_PROFILE_KINDS, float(1 << L.KERNEL_KINDS["residual"]))
+        L.call("bss_set_option", p, L.BSS_OPT_PROFILE_MAX_LEVEL, 0.0)
+        assert L.lib().bss_set_option(p, L.BSS_OPT_PROFILE_KINDS, float(1 << 20)) == -1
+        assert L.lib().bss_set_option(p, L.BSS_OPT_PROFILE_KINDS, 2.5) == -1
+        assert L.lib().bss_set_option(p, L.BSS_OPT_PROFILE_MAX_LEVEL, -1.0) == -1
         v = ctypes.c_int64()
         L.call("bss_plan_get", p, L.BSS_Q_WORKSPACE_BYTES, ctypes.byref(v))
         assert v.value > 0
