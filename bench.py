@@ -351,7 +351,7 @@ def main():
         log(f"bench: {mode}: {r['systems_per_s']:.1f} systems/s, {r['ms_per_step']:.3f} ms/step, "
             f"solve p50 {r['solve_ms_p50']:.3f} ms p99 {r['solve_ms_p99']:.3f} ms, "
             f"iters {r['iterations']}")
-        for (k, lvl), (c, t) in sorted(r["kernels_all"].items(), key=lambda kv: -kv[1][1])[:8]:
+        for (k, lvl), (c, t) in sorted(r["kernels_all"].items(), key=lambda kv: -kv[1][1]):
             log(f"    {k:14s} L{lvl}: {c:4d} launches, {t / c:.4f} ms avg, {t:.3f} ms total")
 
     acc = {}

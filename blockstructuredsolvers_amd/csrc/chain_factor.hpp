@@ -154,7 +154,7 @@ __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* a
 // through diag_chol_inv16 and every panel/trailing/inverse update is an MFMA tile
 // product.  On exit Ubuf's upper tiles hold Uinv = U^{-1}.  Abuf is consumed.
 template <typename T, int NB, int LD, int NW>
-__device__ inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
+__device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
   constexpr int TT = NB / 16;
   typedef typename Mfma<T>::acc_t acc_t;
   const int w = wave_id();
@@ -257,10 +257,17 @@ template <typename T, int NB, int NW>
 constexpr bool factor_prefetch() {
   return factor_min_waves_per_eu<T, NB, NW>() == 1;
 }
+// LAT: the latency variant for levels with at most one chain per CU, where no second
+// workgroup shares the CU: one workgroup per CU, all registers, next blocks prefetched.
+// Identical to the default variant wherever that already runs one workgroup per CU.
+template <typename T, int NB, int NW>
+constexpr bool factor_has_latency_variant() {
+  return factor_min_waves_per_eu<T, NB, NW>() > 1;
+}
 
-template <typename Tin, typename T, int NB, int NW>
+template <typename Tin, typename T, int NB, int NW, bool LAT = false>
 __global__ void __launch_bounds__(64 * NW)
-    __attribute__((amdgpu_waves_per_eu(factor_min_waves_per_eu<T, NB, NW>())))
+    __attribute__((amdgpu_waves_per_eu(LAT ? 1 : factor_min_waves_per_eu<T, NB, NW>())))
     chain_factor_kernel(FactorLevelArgs p) {
   constexpr int TT = NB / 16;
   constexpr int LD = NB + LdsPad<T>::value;
@@ -305,7 +312,7 @@ __global__ void __launch_bounds__(64 * NW)
     sb.load(B + (long long)(start - 1) * NN);
     sb.template store<T, LD, true>(Rbuf + NB * LD);  // Z_1 = B_L^T
   }
-  constexpr bool PF = factor_prefetch<T, NB, NW>();
+  constexpr bool PF = LAT || factor_prefetch<T, NB, NW>();
   if (PF && (len > 1 || hasR)) sb.load(B + (long long)start * NN);
   sa.template store<T, LD, false>(Abuf);
   __syncthreads();

@@ -8,8 +8,30 @@
 
 namespace bss {
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
+}
+
 template <typename Tin, typename T, int NB, int NW>
 static hipError_t factor_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
+  if constexpr (factor_has_latency_variant<T, NB, NW>()) {
+    if (nchains <= device_cus()) {  // one chain per CU at most: latency variant
+      hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW, true>), dim3(nchains),
+                         dim3(64 * NW), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }

@@ -196,9 +196,9 @@ __global__ void __launch_bounds__(4 * NB)
 }
 
 // Combine the partial maxima: berr per (system, column).  `init` records ||d_j||;
-// `mnorm` first reduces ||M||_inf from the same pass.  One 256-thread workgroup per
-// system: each thread folds every 256th partial record, then wave shuffles and LDS
-// finish the reduction.
+// `mnorm` first reduces ||M||_inf from the same pass.  One kRedThreads workgroup per
+// system: each thread folds every kRedThreads-th partial (eight loads in flight), then
+// LDS finishes the reduction.
 constexpr int kRedThreads = 1024;
 
 __global__ void __launch_bounds__(kRedThreads)
@@ -214,7 +214,16 @@ __global__ void __launch_bounds__(kRedThreads)
   const double* base = part + (long long)g0 * kParts;
   const long long total = (long long)(g1 - g0) * kParts;
   double m = 0.0;
-  for (long long i = tid; i < total; i += kRedThreads) m = nmax(m, base[i]);
+  long long i = tid;
+  // the partials were just written by other CUs: issue eight loads before using any
+  for (; i + 7 * kRedThreads < total; i += 8 * kRedThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = base[i + u * kRedThreads];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = nmax(m, v[u]);
+  }
+  for (; i < total; i += kRedThreads) m = nmax(m, base[i]);
   red[tid] = m;
   __syncthreads();
   if (tid < kParts) {

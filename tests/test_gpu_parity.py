@@ -209,6 +209,29 @@ def test_batch_equals_alone(mode):
             1e-13 if mode == "plain" else 1e-12)
 
 
+@pytest.mark.parametrize("mode,T", [("plain", torch.float32), ("mixed", torch.float64)])
+def test_factor_launch_variants_agree(mode, T):
+    """At n = 64 the FP32 factorisation has two launch variants: two chains per CU (a
+    level with more chains than CUs) and one chain per CU with prefetch (fewer).  A
+    system factored alone (75 top-level chains) and inside a batch of eight (600) runs
+    its top level through different variants with the same arithmetic."""
+    systems = [workloads.spd_system(300, 64, nrhs=1, seed=500 + k) for k in range(8)]
+    if T == torch.float32:
+        systems = [workloads.BlockSystem(s.A.astype(np.float32, order="F"),
+                                         s.B.astype(np.float32, order="F"),
+                                         s.d.astype(np.float32, order="F")) for s in systems]
+    xs, _ = run_gpu_batch(systems, T=T, mode=mode, chain_len=3, factor_waves=8, tuned=False)
+    for k in (0, 5):
+        xa, _ = run_gpu(systems[k], T=T, mode=mode, chain_len=3, factor_waves=8,
+                        tuned=False)
+        if mode == "plain":
+            assert np.array_equal(xs[k], xa), k
+        else:
+            assert np.abs(xs[k] - xa).max() <= 1e-12 * np.abs(xa).max()
+            assert oracle.backward_error(systems[k].A, systems[k].B, xs[k],
+                                         systems[k].d).max() <= 1e-12
+
+
 def run_gpu_group(systems, T=torch.float64, **kw):
     from blockstructuredsolvers_amd import initialize_batched_group
     n, nrhs = systems[0].n, systems[0].nrhs
