@@ -252,7 +252,9 @@ constexpr int factor_min_waves_per_eu() {
 }
 // With two workgroups per CU the other chain hides HBM latency, and the registers a
 // prefetch of the next block would hold are better spent not spilling: the next
-// blocks are then loaded when needed (measured 17% faster at FP32, n = 64).
+// blocks are then loaded when needed (measured 17% faster at FP32, n = 64; issuing
+// the loads after the potrf instead, in flight through the trsm only, spilled and was
+// 10-25% slower).
 template <typename T, int NB, int NW>
 constexpr bool factor_prefetch() {
   return factor_min_waves_per_eu<T, NB, NW>() == 1;
