@@ -4,10 +4,11 @@ block-tridiagonal Cholesky solver (the batched GPU path of BlockStructuredSolver
 The compute path is libbss.so (hand-written HIP kernels for gfx950, C ABI in
 include/bss.h); this package is the thin host mirror of the reference API.
 """
-from .solver import (BlockTriDiagData_batched, BlockTriDiagData_group, build_smoother,
-                     factorize, factorize_, initialize_batched, initialize_batched_group,
-                     reference_schedule, solve, solve_)
+from .solver import (BlockTriDiagData_batched, BlockTriDiagData_group, BlockTriDiagData_seq,
+                     build_smoother, factorize, factorize_, initialize_batched,
+                     initialize_batched_group, initialize_seq, reference_schedule, solve,
+                     solve_)
 
-__all__ = ["BlockTriDiagData_batched", "BlockTriDiagData_group", "initialize_batched",
-           "initialize_batched_group", "factorize", "solve", "factorize_", "solve_",
-           "reference_schedule", "build_smoother"]
+__all__ = ["BlockTriDiagData_batched", "BlockTriDiagData_group", "BlockTriDiagData_seq",
+           "initialize_batched", "initialize_batched_group", "initialize_seq", "factorize",
+           "solve", "factorize_", "solve_", "reference_schedule", "build_smoother"]

@@ -314,6 +314,48 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
                                     torch_workspace)
 
 
+class BlockTriDiagData_seq(BlockTriDiagData_batched):
+    """Counterpart of ``BlockTriDiagData_seq`` (src/gpu_seq.jl:1-27): one system
+    eliminated block after block, with no recursion -- the plan's schedule is a single
+    chain of N blocks (one workgroup runs the whole potrf / trsm / Schur-update
+    sequence of ``cholesky_factorize!`` and the substitutions of ``cholesky_solve!``,
+    src/backbone_rocm.jl:14-53).  Besides the tensors and per-block lists it exposes the
+    reference's flat buffers (``A_vec``: all blocks as one (N*n*n, 1) column) and
+    per-block device address arrays (``A_ptrs``, int64, ``device_batch``'s
+    counterpart, src/backbone_rocm.jl:4-7)."""
+
+    def __init__(self, N: int, n: int, T, device, nrhs: int):
+        super().__init__(N, n, T, device, nrhs, 1, "plain", "auto", None, max(N, 1))
+        if self.levels != 1:
+            raise RuntimeError(f"sequential plan has {self.levels} levels, expected 1")
+        es = self._a.element_size()
+
+        def ptrs(t: torch.Tensor, count: int, block: int) -> torch.Tensor:
+            base = t.data_ptr()
+            return torch.tensor([base + i * block * es for i in range(count)],
+                                dtype=torch.int64, device=self.device)
+
+        self.A_vec = self._a.view(-1, 1)
+        self.B_vec = self._b.view(-1, 1)
+        self.d_vec = self._d.view(-1, 1)
+        self.A_ptrs = ptrs(self._a, N, n * n)
+        self.B_ptrs = ptrs(self._b, max(N - 1, 0), n * n)
+        self.d_ptrs = ptrs(self._d, N, n * nrhs)
+
+
+def initialize_seq(N: int, n: int, T=torch.float64, M="cuda", *,
+                   nrhs: int = 1) -> BlockTriDiagData_seq:
+    """``initialize_seq(N, n, T, M)`` (src/gpu_seq.jl:45-59): the sequential solver's
+    data (zeroed A, B, d in the reference layout).  ``factorize``/``solve`` accept it
+    like the batched data.  Extension: ``nrhs`` columns (the reference solves one)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("initialize_seq needs a HIP device; none is visible")
+    device = torch.device(M) if not isinstance(M, torch.device) else M
+    if device.type != "cuda":
+        raise ValueError(f"M must name a GPU device, got {M!r}")
+    return BlockTriDiagData_seq(N, n, T, device, nrhs)
+
+
 def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *,
                              nrhs: int = 1, mode: str = "plain", schedule: str = "auto",
                              chain_len: Optional[int] = None,

@@ -85,3 +85,23 @@ def test_device_smoother_blocks_match_host(n, T):
         Ah = np.asfortranarray(np.repeat(Ai[:, :, None], N, axis=2))
         Bh = np.asfortranarray(np.repeat(Bi[:, :, None], N - 1, axis=2))
         assert oracle.backward_error(Ah, Bh, data.d_tensor.cpu().numpy(), d).max() <= 1e-13
+
+
+def test_serve_frames_round_trip():
+    """Requests sent as wire frames (dense and model-carrying) come back as result frames
+    whose solutions equal the same requests solved alone, bitwise (plain FP64)."""
+    from blockstructuredsolvers_amd import wire
+    reqs = small_stream(4, rate=0.0, seed=21) + small_stream(3, rate=0.0, seed=22,
+                                                             device_blocks=True)
+    for k, r in enumerate(reqs):
+        r.rid = 1000 + k
+    frames = [wire.encode_request(r) for r in reqs]
+    srv = serving.Server(mode="plain", workers=4, tuned=False)
+    out_frames, records = wire.serve_frames(srv, frames, realtime=False)
+    assert [rec["rid"] for rec in records] == [r.rid for r in reqs]
+    for r, fr in zip(reqs, out_frames):
+        rid, info, xs = wire.decode_result(fr)
+        assert rid == r.rid and info == 0 and len(xs) == len(r.rhs)
+        alone = serving.solve_alone(r, mode="plain", tuned=False)
+        for xs_k, xa in zip(xs, alone):
+            assert np.array_equal(xs_k, xa), r.rid
