@@ -74,18 +74,17 @@ __device__ inline float row_bcast(float x) {
       __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, 0xf, 0xf, false));
 }
 
-// 1/sqrt(p): hardware estimate refined by Newton steps (two for f64, one for f32),
-// avoiding the long correctly-rounded sqrt + divide sequences on the serial path.
+// 1/sqrt(p) on the serial path, avoiding the long correctly-rounded sqrt + divide
+// sequences: f64 refines the hardware estimate with two Newton steps; f32 takes
+// v_rsq_f32 as is (1 ulp), the FP32 factor's own rounding level (a Newton step there
+// measured 2% of the FP32 factor time and no accuracy change in the refined result).
 __device__ inline double rsqrt_nr(double p) {
   double y = __builtin_amdgcn_rsq(p);
   y = y * (1.5 - 0.5 * p * y * y);
   y = y * (1.5 - 0.5 * p * y * y);
   return y;
 }
-__device__ inline float rsqrt_nr(float p) {
-  float y = __builtin_amdgcn_rsqf(p);
-  return y * (1.5f - 0.5f * p * y * y);
-}
+__device__ inline float rsqrt_nr(float p) { return __builtin_amdgcn_rsqf(p); }
 
 // One elimination step J of the 16x16 tile (see diag_chol_inv16).  Lane c (of every
 // 16-lane row; the four rows compute the same thing) holds column c: a[r] = A[r][c],

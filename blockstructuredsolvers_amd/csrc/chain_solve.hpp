@@ -154,9 +154,13 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   ou.load(U + (long long)start * NN + 16 * w * NB, NB, 0, w + 1);
   if (sepL >= 0) ow.load(W + (long long)start * NN + 16 * w * NB, NB);
   acc_t accL = Mfma<T>::zero();
+  // the right-hand side tile of the next block is loaded one block ahead (below the
+  // top level it is three dependent loads through the hand-off index)
+  acc_t rnext = rhs_tile(start);
   for (int i = 0; i < len; ++i) {
     const long long blk = start + i;
-    acc_t acc = rhs_tile(blk);
+    acc_t acc = rnext;
+    if (i + 1 < len) rnext = rhs_tile(blk + 1);
     if (i > 0) oc.template mma<true>(acc, Y, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
     if (i + 1 < len) oc.load(C + blk * NN + 16 * w * NB, NB);  // C_blk, used by row blk + 1
@@ -230,9 +234,11 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
     if (sepL >= 0) ow.load(W + blk * NN + 16 * w, NB);
     ou.load(U + blk * NN + 16 * w, NB, w, TT);
   }
+  acc_t rnext = load_rhs_tile<T, NB>(d + (long long)(start + len - 1) * ND, w, nrhs);
   for (int i = len - 1; i >= 0; --i) {
     const long long blk = start + i;
-    acc_t acc = load_rhs_tile<T, NB>(d + blk * ND, w, nrhs);
+    acc_t acc = rnext;  // y_blk; the tile of block blk - 1 is loaded one block ahead
+    if (i > 0) rnext = load_rhs_tile<T, NB>(d + (blk - 1) * ND, w, nrhs);
     if (i < len - 1 || sepR >= 0) oc.template mma<true>(acc, Xn, LDV);
     if (sepL >= 0) ow.template mma<true>(acc, XL, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);  // consumes the products above
