@@ -37,8 +37,8 @@ __device__ inline void zero_vec(T* v) {
 }
 
 // d-tile (rows 16w.., all rhs columns) from a global n x nrhs column-major block
-template <typename T, int NB>
-__device__ inline typename Mfma<T>::acc_t load_rhs_tile(const T* g, int w, int nrhs) {
+template <typename T, int NB, typename P>
+__device__ inline typename Mfma<T>::acc_t load_rhs_tile(P g, int w, int nrhs) {
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
   typename Mfma<T>::acc_t a;
@@ -48,8 +48,8 @@ __device__ inline typename Mfma<T>::acc_t load_rhs_tile(const T* g, int w, int n
   return a;
 }
 
-template <typename T, int NB>
-__device__ inline void store_rhs_tile(T* g, int w, int nrhs, const typename Mfma<T>::acc_t& a) {
+template <typename T, int NB, typename P>
+__device__ inline void store_rhs_tile(P g, int w, int nrhs, const typename Mfma<T>::acc_t& a) {
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
   if (lr < nrhs) {
@@ -59,8 +59,8 @@ __device__ inline void store_rhs_tile(T* g, int w, int nrhs, const typename Mfma
 }
 
 // global n x nrhs block -> LDS vector buffer (columns >= nrhs left untouched)
-template <typename T, int NB, int LDV, int NT>
-__device__ inline void load_vec(const T* g, T* v, int nrhs) {
+template <typename T, int NB, int LDV, int NT, typename P>
+__device__ inline void load_vec(P g, T* v, int nrhs) {
   for (int i = threadIdx.x; i < NB * nrhs; i += NT) v[(i / NB) * LDV + (i % NB)] = g[i];
 }
 
@@ -70,7 +70,8 @@ __device__ inline void load_vec(const T* g, T* v, int nrhs) {
 template <typename T, int NB, bool TA>
 struct XOps {
   T a[NB / 16][4];
-  __device__ inline void load(const T* __restrict__ xb, int ldx, int s_lo = 0,
+  template <typename P>
+  __device__ inline void load(P __restrict__ xb, int ldx, int s_lo = 0,
                               int s_hi = NB / 16) {
     const int l = lane_id();
     const int lr = l & 15, lk = l >> 4;
@@ -115,19 +116,19 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
-  const T* U = (const T*)S.U;
-  const T* C = (const T*)S.C;
-  const T* W = (const T*)S.W;
-  T* d = (T*)S.d;
+  const auto U = gptr((const T*)S.U);
+  const auto C = gptr((const T*)S.C);
+  const auto W = gptr((const T*)S.W);
+  const auto d = gptr((T*)S.d);
   const int w = wave_id();
   // right-hand side of block q: below the top level (fused hand-off) it is assembled
   // from the previous level, d[q] = dp[pp[q]] - rRp[q] - rLp[q]
   const bool assemble = S.fused && S.dp;
   auto rhs_tile = [&](long long q) -> acc_t {
     if (!assemble) return load_rhs_tile<T, NB>(d + q * ND, w, nrhs);
-    acc_t a = load_rhs_tile<T, NB>((const T*)S.dp + (long long)S.pp[q] * ND, w, nrhs);
-    const acc_t r = load_rhs_tile<T, NB>((const T*)S.rRp + q * ND, w, nrhs);
-    const acc_t l = load_rhs_tile<T, NB>((const T*)S.rLp + q * ND, w, nrhs);
+    acc_t a = load_rhs_tile<T, NB>(gptr((const T*)S.dp) + (long long)gptr(S.pp)[q] * ND, w, nrhs);
+    const acc_t r = load_rhs_tile<T, NB>(gptr((const T*)S.rRp) + q * ND, w, nrhs);
+    const acc_t l = load_rhs_tile<T, NB>(gptr((const T*)S.rLp) + q * ND, w, nrhs);
 #pragma unroll
     for (int v = 0; v < 4; ++v) a[v] = a[v] - r[v] - l[v];
     return a;
@@ -180,9 +181,9 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
     const long long last = start + len - 1;
     acc_t acc = Mfma<T>::zero();
     tile_mma<T, true, false, false, NB>(acc, C + last * NN + 16 * w * NB, NB, Y, LDV);
-    store_rhs_tile<T, NB>((T*)S.rR + sepR * ND, w, nrhs, acc);
+    store_rhs_tile<T, NB>(gptr((T*)S.rR) + sepR * ND, w, nrhs, acc);
   }
-  if (sepL >= 0) store_rhs_tile<T, NB>((T*)S.rL + sepL * ND, w, nrhs, accL);
+  if (sepL >= 0) store_rhs_tile<T, NB>(gptr((T*)S.rL) + sepL * ND, w, nrhs, accL);
 }
 
 template <typename T, int NB>
@@ -201,11 +202,11 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
   const LevelSys S = p.sys[cd.slot];
   const int nrhs = p.nrhs;
   const long long ND = (long long)NB * nrhs;
-  const T* U = (const T*)S.U;
-  const T* C = (const T*)S.C;
-  const T* W = (const T*)S.W;
-  T* d = (T*)S.d;
-  const T* xs = (const T*)S.dn;
+  const auto U = gptr((const T*)S.U);
+  const auto C = gptr((const T*)S.C);
+  const auto W = gptr((const T*)S.W);
+  const auto d = gptr((T*)S.d);
+  const auto xs = gptr((const T*)S.dn);
   const int w = wave_id();
   // fused hand-off: the solutions of the separators this chain owns (as in the forward
   // sweep) into this level's vector, for the level above or the caller at the top

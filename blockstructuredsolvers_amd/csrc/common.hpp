@@ -62,6 +62,16 @@ __device__ inline float rsqrt_t<float>(float x) {
   return 1.0f / sqrtf(x);
 }
 
+// Pointers read from device tables are generic, and loads through them compile to FLAT
+// memory instructions, which count against the LDS wait counter as well: every wait for
+// an LDS read then also waits for all HBM loads in flight, defeating prefetches.
+// Kernels reach HBM through global-address-space pointers instead (gptr).
+#define BSS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __attribute__((always_inline)) inline BSS_GLOBAL T* gptr(T* p) {
+  return (BSS_GLOBAL T*)p;
+}
+
 // acc += op(X) * op(Y) for one 16x16 output tile, K a multiple of 16 (compile-time, so
 // the whole product is unrolled and every operand load issues before the MFMAs).
 //   TA: op(X) = X^T  -> P[i][k] = X[k][i] at xb[i*ldx + k]; else X[i][k] at xb[k*ldx + i]
@@ -69,9 +79,9 @@ __device__ inline float rsqrt_t<float>(float x) {
 //   NEG: subtract instead of add (the A operand is negated).
 // xb/yb point at element (0,0) of the operands' 16-row/col panels; they may live in
 // LDS or global memory.
-template <typename T, bool TA, bool TB, bool NEG, int K>
-__device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
-                                int ldx, const T* __restrict__ yb, int ldy) {
+template <typename T, bool TA, bool TB, bool NEG, int K, typename PX, typename PY>
+__device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, PX __restrict__ xb, int ldx,
+                                PY __restrict__ yb, int ldy) {
   static_assert(K % 16 == 0, "K must be a multiple of 16");
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
@@ -91,9 +101,9 @@ __device__ inline void tile_mma(typename Mfma<T>::acc_t& acc, const T* __restric
 
 // acc += op(X) op(Y) over K in 16-deep steps unrolled by two (not fully), which keeps
 // the operand registers of a full-depth product bounded
-template <typename T, bool TA, bool TB, bool NEG, int K>
-__device__ inline void tile_mma_k(typename Mfma<T>::acc_t& acc, const T* __restrict__ xb,
-                                  int ldx, const T* __restrict__ yb, int ldy) {
+template <typename T, bool TA, bool TB, bool NEG, int K, typename PX, typename PY>
+__device__ inline void tile_mma_k(typename Mfma<T>::acc_t& acc, PX __restrict__ xb, int ldx,
+                                  PY __restrict__ yb, int ldy) {
 #pragma unroll 2
   for (int k0 = 0; k0 < K; k0 += 16)
     tile_mma<T, TA, TB, NEG, 16>(acc, TA ? xb + k0 : xb + (long long)k0 * ldx, ldx,

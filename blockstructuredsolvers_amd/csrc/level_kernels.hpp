@@ -25,10 +25,10 @@ __global__ void __launch_bounds__(256) assemble_diag_kernel(const SepRec* __rest
                                                              int nn) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
-  const Tin* a = (const Tin*)S.A + (long long)sr.pos * nn;
-  const Ts* r = (const Ts*)S.SR + (long long)sr.p * nn;
-  const Ts* l = (const Ts*)S.SL + (long long)sr.p * nn;
-  To* an = (To*)S.An + (long long)sr.p * nn;
+  const auto a = gptr((const Tin*)S.A) + (long long)sr.pos * nn;
+  const auto r = gptr((const Ts*)S.SR) + (long long)sr.p * nn;
+  const auto l = gptr((const Ts*)S.SL) + (long long)sr.p * nn;
+  const auto an = gptr((To*)S.An) + (long long)sr.p * nn;
   for (int e = threadIdx.x; e < nn; e += blockDim.x) {
     const double v = (double)a[e] - (double)r[e] - (double)l[e];
     an[e] = (To)v;
@@ -41,10 +41,10 @@ __global__ void __launch_bounds__(256) assemble_rhs_kernel(const SepRec* __restr
                                                             int nd) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
-  const Ts* src = (const Ts*)S.d + (long long)sr.pos * nd;
-  const Ts* a = (const Ts*)S.rR + (long long)sr.p * nd;
-  const Ts* b = (const Ts*)S.rL + (long long)sr.p * nd;
-  To* dst = (To*)S.dn + (long long)sr.p * nd;
+  const auto src = gptr((const Ts*)S.d) + (long long)sr.pos * nd;
+  const auto a = gptr((const Ts*)S.rR) + (long long)sr.p * nd;
+  const auto b = gptr((const Ts*)S.rL) + (long long)sr.p * nd;
+  const auto dst = gptr((To*)S.dn) + (long long)sr.p * nd;
   for (int e = threadIdx.x; e < nd; e += blockDim.x) {
     if constexpr (sizeof(Ts) == sizeof(To))
       dst[e] = src[e] - a[e] - b[e];
@@ -59,8 +59,8 @@ __global__ void __launch_bounds__(256) scatter_sep_kernel(const SepRec* __restri
                                                            int nd) {
   const SepRec sr = seps[blockIdx.x];
   const LevelSys& S = sys[sr.slot];
-  const To* src = (const To*)S.dn + (long long)sr.p * nd;
-  Ts* dst = (Ts*)S.d + (long long)sr.pos * nd;
+  const auto src = gptr((const To*)S.dn) + (long long)sr.p * nd;
+  const auto dst = gptr((Ts*)S.d) + (long long)sr.pos * nd;
   for (int e = threadIdx.x; e < nd; e += blockDim.x) dst[e] = (Ts)src[e];
 }
 
@@ -69,9 +69,11 @@ __global__ void __launch_bounds__(256) scatter_sep_kernel(const SepRec* __restri
 template <typename S, typename D>
 __global__ void __launch_bounds__(256) convert_kernel(const ConvRec<S, D>* __restrict__ recs) {
   const ConvRec<S, D> r = recs[blockIdx.x];
+  const auto src = gptr(r.src);
+  const auto dst = gptr(r.dst);
   for (long long e = (long long)blockIdx.y * blockDim.x + threadIdx.x; e < r.count;
        e += (long long)gridDim.y * blockDim.x)
-    r.dst[e] = (D)r.src[e];
+    dst[e] = (D)src[e];
 }
 
 }  // namespace bss

@@ -49,13 +49,13 @@ __global__ void __launch_bounds__(4 * NB)
   const int tid = threadIdx.x;
   const int w = wave_id(), l = lane_id(), lr = l & 15, lk = l >> 4;
   const long long ND = (long long)NB * nrhs;
-  const double* As = S.A;
-  const double* Bs = S.B;
-  const double* ds = S.d;
-  const double* xo = xo_k >= 0 ? S.x[xo_k] : nullptr;
-  const float* dxs = dx_k >= 0 ? S.v[dx_k] : nullptr;
-  double* xn = S.x[xn_k];
-  float* ro = S.v[ro_k];
+  const auto As = gptr((const double*)S.A);
+  const auto Bs = gptr((const double*)S.B);
+  const auto ds = gptr((const double*)S.d);
+  const auto xo = gptr((const double*)(xo_k >= 0 ? S.x[xo_k] : nullptr));
+  const auto dxs = gptr((const float*)(dx_k >= 0 ? S.v[dx_k] : nullptr));
+  const auto xn = gptr(S.x[xn_k]);
+  const auto ro = gptr(S.v[ro_k]);
   for (int e = tid; e < 3 * 16 * LDV; e += NT) (&X[0][0])[e] = 0.0;
   double rmax = 0.0, xmax = 0.0, dmax = 0.0, mmax = 0.0;
   const int b0 = it.b0, b1 = it.b1;
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kRedThreads)
   const int g0 = systems[sys].item0, g1 = systems[sys].item1;
   // the system's partials are contiguous (item-major, kParts slots each): thread t
   // reads every kRedThreads-th value, all of slot t % kParts, coalesced
-  const double* base = part + (long long)g0 * kParts;
+  const auto base = gptr(part) + (long long)g0 * kParts;
   const long long total = (long long)(g1 - g0) * kParts;
   double m = 0.0;
   long long i = tid;
