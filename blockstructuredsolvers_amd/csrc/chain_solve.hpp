@@ -41,10 +41,15 @@ template <typename T, int NB, typename P>
 __device__ inline typename Mfma<T>::acc_t load_rhs_tile(P g, int w, int nrhs) {
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
+  // every lane loads (columns past nrhs read column 0 and are zeroed): a predicated load
+  // would make the wait counters conservative for every later load
+  const int c = lr < nrhs ? lr : 0;
   typename Mfma<T>::acc_t a;
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
-    a[v] = (lr < nrhs) ? g[lr * NB + 16 * w + Mfma<T>::row(lk, v)] : T(0);
+  for (int v = 0; v < 4; ++v) {
+    const T x = g[c * NB + 16 * w + Mfma<T>::row(lk, v)];
+    a[v] = (lr < nrhs) ? x : T(0);
+  }
   return a;
 }
 

@@ -49,7 +49,9 @@ struct Mfma<float> {
 };
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
-__device__ inline int wave_id() { return threadIdx.x >> 6; }
+// wave index, in a scalar register: branches on it are uniform, so loads under them stay
+// unpredicated and the wait counters can be tracked exactly
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 template <typename T>
 __device__ inline T rsqrt_t(T x);
