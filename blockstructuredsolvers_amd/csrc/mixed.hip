@@ -60,19 +60,45 @@ __global__ void __launch_bounds__(4 * NB)
   double rmax = 0.0, xmax = 0.0, dmax = 0.0, mmax = 0.0;
   const int b0 = it.b0, b1 = it.b1;
 
-  // x_new of block j into its window slot (zero outside the system); own blocks are
-  // also written to xnew
-  auto stage = [&](int j) {
-    double* slot = X[((j % 3) + 3) % 3];
-    for (int e = tid; e < (int)ND; e += NT) {
-      double v = 0.0;
-      if (!INIT && j >= 0 && j < N) {
-        if (xo) v = xo[(long long)j * ND + e];
-        if (dxs) v += (double)dxs[(long long)j * ND + e];
+  // x_new of block j (zero outside the system), elements e = tid + u*NT: loaded into
+  // registers by stage_load, then written to its window slot (and, for the
+  // workgroup's own blocks, to xnew) by stage_store.  The wait counters are in order,
+  // so the x loads are issued before the block's operand loads, and the wait for them
+  // leaves the operands in flight; every lane loads (clamped indices), as a predicated
+  // load would make every later wait conservative.  Past the initial pass x_old and
+  // dx are always given.
+  constexpr int SE = (NB * 16 + NT - 1) / NT;  // up to 16 right-hand sides
+  auto stage_load = [&](int j, double (&v)[SE]) {
+    const bool inside = j >= 0 && j < N;
+    const long long jc = j < 0 ? 0 : (j >= N ? N - 1 : j);
+#pragma unroll
+    for (int u = 0; u < SE; ++u) {
+      if constexpr (INIT) {
+        v[u] = 0.0;
+      } else {
+        const int e = tid + u * NT;
+        const long long o = jc * ND + (e < (int)ND ? e : 0);
+        const double x = xo[o] + (double)dxs[o];
+        v[u] = inside ? x : 0.0;
       }
-      slot[(e / NB) * LDV + (e % NB)] = v;
-      if (j >= b0 && j < b1) xn[(long long)j * ND + e] = v;
     }
+  };
+  auto stage_store = [&](int j, const double (&v)[SE]) {
+    double* slot = X[((j % 3) + 3) % 3];
+    const bool own = j >= b0 && j < b1;
+#pragma unroll
+    for (int u = 0; u < SE; ++u) {
+      const int e = tid + u * NT;
+      if (e < (int)ND) {
+        slot[(e / NB) * LDV + (e % NB)] = v[u];
+        if (own) xn[(long long)j * ND + e] = v[u];
+      }
+    }
+  };
+  auto stage = [&](int j) {
+    double v[SE];
+    stage_load(j, v);
+    stage_store(j, v);
   };
   // k index of operand slot q: 4 consecutive k per lane (contiguous loads along k)
   auto kof = [&](int q) { return (q >> 2) * 16 + 4 * lk + (q & 3); };
@@ -112,28 +138,40 @@ __global__ void __launch_bounds__(4 * NB)
     // for row blk + 1 when that row is ours)
     const bool next = blk + 1 < N;
     const bool carry_on = next && blk + 1 < b1;
+    double xs[SE];
+    stage_load(blk + 1, xs);
     double aA[KC], aB[KC], aBn[KC];
+    // B_blk exists only below the last block; otherwise its loads read A_blk (always
+    // there) and are zeroed, so no load is conditional
+    const auto Bb = next ? Bs + blk * NN : As + blk * NN;
     auto load_chunk = [&](int ch) {
 #pragma unroll
       for (int qq = 0; qq < KC; ++qq) {
         const int k = kof(ch * KC + qq);
         aA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
-        aB[qq] = next ? Bs[blk * NN + (long long)k * NB + 16 * w + lr] : 0.0;
+        const double b = Bb[(long long)k * NB + 16 * w + lr];
+        aB[qq] = next ? b : 0.0;
       }
 #pragma unroll
-      for (int qq = 0; qq < KC; ++qq)
-        aBn[qq] = carry_on ? Bs[blk * NN + (long long)(16 * w + lr) * NB + kof(ch * KC + qq)] : 0.0;
+      for (int qq = 0; qq < KC; ++qq) {
+        const double b = Bb[(long long)(16 * w + lr) * NB + kof(ch * KC + qq)];
+        aBn[qq] = carry_on ? b : 0.0;
+      }
     };
-    if (!INIT) load_chunk(0);
+    // d_blk, like x, ahead of the operands
     acc_t acc;
+    const int dc = lr < nrhs ? lr : 0;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = 16 * w + Mfma<double>::row(lk, v);
-      acc[v] = (lr < nrhs) ? ds[(long long)blk * ND + lr * NB + row] : 0.0;
-      dmax = nmax(dmax, fabs(acc[v]));
+      const double x = ds[(long long)blk * ND + dc * NB + row];
+      acc[v] = (lr < nrhs) ? x : 0.0;
     }
+    if (!INIT) load_chunk(0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dmax = nmax(dmax, fabs(acc[v]));
     __syncthreads();  // previous block's window reads are done
-    stage(blk + 1);
+    stage_store(blk + 1, xs);
     __syncthreads();
     const double* X0 = X[blk % 3];
     const double* Xp = X[(blk + 1) % 3];
