@@ -37,20 +37,29 @@ __device__ inline void zero_vec(T* v) {
 }
 
 // d-tile (rows 16w.., all rhs columns) from a global n x nrhs column-major block
+// Every lane loads (columns past nrhs read column 0): a predicated load would make the
+// in-order wait counters conservative for every later load.  load_rhs_raw leaves those
+// columns as loaded, so a tile loaded ahead is not waited for until mask_rhs uses it.
 template <typename T, int NB, typename P>
-__device__ inline typename Mfma<T>::acc_t load_rhs_tile(P g, int w, int nrhs) {
+__device__ inline typename Mfma<T>::acc_t load_rhs_raw(P g, int w, int nrhs) {
   const int l = lane_id();
   const int lr = l & 15, lk = l >> 4;
-  // every lane loads (columns past nrhs read column 0 and are zeroed): a predicated load
-  // would make the wait counters conservative for every later load
   const int c = lr < nrhs ? lr : 0;
   typename Mfma<T>::acc_t a;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const T x = g[c * NB + 16 * w + Mfma<T>::row(lk, v)];
-    a[v] = (lr < nrhs) ? x : T(0);
-  }
+  for (int v = 0; v < 4; ++v) a[v] = g[c * NB + 16 * w + Mfma<T>::row(lk, v)];
   return a;
+}
+template <typename T>
+__device__ inline typename Mfma<T>::acc_t mask_rhs(typename Mfma<T>::acc_t a, int nrhs) {
+  const bool in = (lane_id() & 15) < nrhs;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a[v] = in ? a[v] : T(0);
+  return a;
+}
+template <typename T, int NB, typename P>
+__device__ inline typename Mfma<T>::acc_t load_rhs_tile(P g, int w, int nrhs) {
+  return mask_rhs<T>(load_rhs_raw<T, NB>(g, w, nrhs), nrhs);
 }
 
 template <typename T, int NB, typename P>
@@ -80,12 +89,16 @@ struct XOps {
                               int s_hi = NB / 16) {
     const int l = lane_id();
     const int lr = l & 15, lk = l >> 4;
+    // slices outside [s_lo, s_hi) re-read an in-range slice (a cache hit) and are zeroed:
+    // a fixed number of loads per call keeps the in-order wait counters exact
 #pragma unroll
     for (int s = 0; s < NB / 16; ++s)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = 16 * s + 4 * lk + q;
-        a[s][q] = (s >= s_lo && s < s_hi) ? (TA ? xb[lr * ldx + k] : xb[k * ldx + lr]) : T(0);
+        const int sc = s < s_lo ? s_lo : (s >= s_hi ? s_hi - 1 : s);
+        const int k = 16 * sc + 4 * lk + q;
+        const T x = TA ? xb[lr * ldx + k] : xb[k * ldx + lr];
+        a[s][q] = (s >= s_lo && s < s_hi) ? x : T(0);
       }
   }
   // acc (+/-)= op(X) Y over the slices [s_lo, s_hi), Y[k][j] at yb[j*ldy + k] (LDS)
@@ -129,15 +142,30 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   // right-hand side of block q: below the top level (fused hand-off) it is assembled
   // from the previous level, d[q] = dp[pp[q]] - rRp[q] - rLp[q]
   const bool assemble = S.fused && S.dp;
-  auto rhs_tile = [&](long long q) -> acc_t {
-    if (!assemble) return load_rhs_tile<T, NB>(d + q * ND, w, nrhs);
-    acc_t a = load_rhs_tile<T, NB>(gptr((const T*)S.dp) + (long long)gptr(S.pp)[q] * ND, w, nrhs);
-    const acc_t r = load_rhs_tile<T, NB>(gptr((const T*)S.rRp) + q * ND, w, nrhs);
-    const acc_t l = load_rhs_tile<T, NB>(gptr((const T*)S.rLp) + q * ND, w, nrhs);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) a[v] = a[v] - r[v] - l[v];
-    return a;
+  // the loaded parts of a right-hand side tile, combined only where the tile is used
+  // (so a tile loaded one block ahead does not make this block wait for it)
+  struct RhsParts {
+    acc_t a, r, l;
   };
+  auto rhs_parts = [&](long long q) -> RhsParts {
+    RhsParts t;
+    if (!assemble) {
+      t.a = load_rhs_raw<T, NB>(d + q * ND, w, nrhs);
+    } else {
+      t.a = load_rhs_raw<T, NB>(gptr((const T*)S.dp) + (long long)gptr(S.pp)[q] * ND, w, nrhs);
+      t.r = load_rhs_raw<T, NB>(gptr((const T*)S.rRp) + q * ND, w, nrhs);
+      t.l = load_rhs_raw<T, NB>(gptr((const T*)S.rLp) + q * ND, w, nrhs);
+    }
+    return t;
+  };
+  auto rhs_combine = [&](const RhsParts& t) -> acc_t {
+    if (!assemble) return mask_rhs<T>(t.a, nrhs);
+    acc_t a;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] = t.a[v] - t.r[v] - t.l[v];
+    return mask_rhs<T>(a, nrhs);
+  };
+  auto rhs_tile = [&](long long q) -> acc_t { return rhs_combine(rhs_parts(q)); };
   // the separators this chain owns get their assembled right-hand side, for the next
   // level: the one on its right, and a separator at block 0 (which has no chain on its
   // left; the reference layout starts a level with one)
@@ -162,11 +190,11 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   acc_t accL = Mfma<T>::zero();
   // the right-hand side tile of the next block is loaded one block ahead (below the
   // top level it is three dependent loads through the hand-off index)
-  acc_t rnext = rhs_tile(start);
+  RhsParts rnext = rhs_parts(start);
   for (int i = 0; i < len; ++i) {
     const long long blk = start + i;
-    acc_t acc = rnext;
-    if (i + 1 < len) rnext = rhs_tile(blk + 1);
+    acc_t acc = rhs_combine(rnext);
+    if (i + 1 < len) rnext = rhs_parts(blk + 1);
     if (i > 0) oc.template mma<true>(acc, Y, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);
     if (i + 1 < len) oc.load(C + blk * NN + 16 * w * NB, NB);  // C_blk, used by row blk + 1
@@ -240,11 +268,11 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
     if (sepL >= 0) ow.load(W + blk * NN + 16 * w, NB);
     ou.load(U + blk * NN + 16 * w, NB, w, TT);
   }
-  acc_t rnext = load_rhs_tile<T, NB>(d + (long long)(start + len - 1) * ND, w, nrhs);
+  acc_t rnext = load_rhs_raw<T, NB>(d + (long long)(start + len - 1) * ND, w, nrhs);
   for (int i = len - 1; i >= 0; --i) {
     const long long blk = start + i;
-    acc_t acc = rnext;  // y_blk; the tile of block blk - 1 is loaded one block ahead
-    if (i > 0) rnext = load_rhs_tile<T, NB>(d + (blk - 1) * ND, w, nrhs);
+    acc_t acc = mask_rhs<T>(rnext, nrhs);  // y_blk; block blk - 1's tile loads one ahead
+    if (i > 0) rnext = load_rhs_raw<T, NB>(d + (blk - 1) * ND, w, nrhs);
     if (i < len - 1 || sepR >= 0) oc.template mma<true>(acc, Xn, LDV);
     if (sepL >= 0) ow.template mma<true>(acc, XL, LDV);
     tile_store<T>(R + 16 * w, LDV, acc);  // consumes the products above
