@@ -313,6 +313,8 @@ __global__ void __launch_bounds__(64 * NW)
     sb.load(B + (long long)(start - 1) * NN);
     sb.template store<T, LD, true>(Rbuf + NB * LD);  // Z_1 = B_L^T
   }
+  int fail_i = len;       // first failed block of the chain (thread 0)
+  float piv_run = 1.0f;   // smallest pivot ratio of the chain (wave 0)
   constexpr bool PF = LAT || factor_prefetch<T, NB, NW>();
   if (PF && (len > 1 || hasR)) sb.load(B + (long long)start * NN);
   sa.template store<T, LD, false>(Abuf);
@@ -334,9 +336,11 @@ __global__ void __launch_bounds__(64 * NW)
     float adiag = 0.0f;
     if (p.pivmin && w == 0 && lane_id() < NB) adiag = (float)Abuf[lane_id() * (LD + 1)];
     potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
+    // failures and pivot ratios are kept per workgroup and reported once after the
+    // chain: a global atomic inside the loop, on a path some lanes skip, would make the
+    // memory waits after it conservative (and drain the prefetches)
     if (tid == 0 && fail) {
-      const int ob = S.orig ? S.orig[blk] : blk;
-      atomicMin(p.info + sys, ob + 1);
+      fail_i = min(fail_i, i);
       fail = 0;
     }
     if (p.pivmin && w == 0) {
@@ -350,7 +354,7 @@ __global__ void __launch_bounds__(64 * NW)
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) ratio = fminf(ratio, __shfl_xor(ratio, o));
-      if (lane_id() == 0) atomicMin(p.pivmin + sys, __float_as_uint(ratio));
+      piv_run = fminf(piv_run, ratio);
     }
 
     // [C | W] = Uinv^T [B | Z].  Output tile (r, c) costs r+1 tile products; wave w
@@ -499,6 +503,13 @@ __global__ void __launch_bounds__(64 * NW)
       }
     }
   }
+  // the chain's first failed block (original indices increase along a chain, so the
+  // first local failure is the smallest original index) and its smallest pivot ratio
+  if (tid == 0 && fail_i < len) {
+    const int blk = start + fail_i;
+    atomicMin(p.info + sys, (S.orig ? S.orig[blk] : blk) + 1);
+  }
+  if (p.pivmin && tid == 0) atomicMin(p.pivmin + sys, __float_as_uint(piv_run));
 }
 
 }  // namespace bss

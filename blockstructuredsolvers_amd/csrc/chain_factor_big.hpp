@@ -89,6 +89,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
 #pragma unroll
   for (int q = 0; q < SQ; ++q) Sacc[q] = Mfma<T>::zero();
   if (tid == 0) fail = 0;
+  int fail_i = len;      // first failed block of the chain (thread 0)
+  float piv_run = 1.0f;  // smallest pivot ratio of the chain (wave 0)
 
   if (hasL) {  // Z_1 = B_L^T into the first block's panel slot
     const Tin* bl = B + (long long)(start - 1) * NN;
@@ -153,9 +155,9 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
       }
       if (nrow > 0) __syncthreads();
     }
+    // reported once after the chain (see chain_factor.hpp)
     if (tid == 0 && fail) {
-      const int ob = S.orig ? S.orig[blk] : (int)blk;
-      atomicMin(p.info + sys, ob + 1);
+      fail_i = min(fail_i, i);
       fail = 0;
     }
     if (p.pivmin && w == 0) {
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) ratio = fminf(ratio, __shfl_xor(ratio, o));
-      if (lane_id() == 0) atomicMin(p.pivmin + sys, __float_as_uint(ratio));
+      piv_run = fminf(piv_run, ratio);
     }
     // 3. trtri in place, column by column: Uinv(0:j, j) = -Uinv(0:j, 0:j) U(0:j, j) Uinv_jj
     for (int j = 1; j < TT; ++j) {
@@ -278,6 +280,11 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
       }
     }
   }
+  if (tid == 0 && fail_i < len) {
+    const long long blk = start + fail_i;
+    atomicMin(p.info + sys, (S.orig ? S.orig[blk] : (int)blk) + 1);
+  }
+  if (p.pivmin && tid == 0) atomicMin(p.pivmin + sys, __float_as_uint(piv_run));
 }
 
 }  // namespace bss
