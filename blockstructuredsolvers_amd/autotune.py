@@ -58,7 +58,7 @@ def lookup(N: int, n: int, dtype, mode: str, path: str = CACHE) -> Optional[dict
 def candidates(n: int) -> List[dict]:
     waves = [4, 8] if n == 64 else [None]
     out = []
-    for w, top, low, base in itertools.product(waves, [2, 3, 4, 6], [1, 2, 3], [2, 4, 8]):
+    for w, top, low, base in itertools.product(waves, [2, 3, 4, 6], [1, 2, 3, 4], [2, 4, 8]):
         out.append({"factor_waves": w, "chain_len_levels": [top] + [low] * 15,
                     "base_max": base})
     return out
@@ -67,12 +67,16 @@ def candidates(n: int) -> List[dict]:
 def _time_config(cfg, systems, dtype, mode, reps, bss):
     from .solver import initialize_batched
     s0 = systems[0]
-    N, n, nrhs = s0.N, s0.n, s0.nrhs
-    data = initialize_batched(N, n, dtype, "cuda", nrhs=nrhs, nsys=1, mode=mode,
+    N, n, nrhs, S = s0.N, s0.n, s0.nrhs, len(systems)
+    data = initialize_batched(N, n, dtype, "cuda", nrhs=nrhs, nsys=S, mode=mode,
                               preserve_input=(mode == "plain"), **cfg)
-    data.A_tensor.copy_(torch.from_numpy(s0.A).to(dtype))
-    data.B_tensor.copy_(torch.from_numpy(s0.B).to(dtype))
-    d0 = torch.from_numpy(s0.d).to(dtype).cuda()
+    for k, sk in enumerate(systems):
+        view = (lambda t: t) if S == 1 else (lambda t, k=k: t[..., k])
+        view(data.A_tensor).copy_(torch.from_numpy(sk.A).to(dtype))
+        view(data.B_tensor).copy_(torch.from_numpy(sk.B).to(dtype))
+    d0 = torch.stack([torch.from_numpy(sk.d).to(dtype) for sk in systems], dim=-1).cuda()
+    if S == 1:
+        d0 = d0[..., 0]
     ts = []
     for r in range(reps + 1):
         data.d_tensor.copy_(d0)
@@ -89,11 +93,12 @@ def _time_config(cfg, systems, dtype, mode, reps, bss):
 
 def tune(N: int, n: int, nrhs: int = 1, dtype=torch.float64, mode: str = "plain",
          reps: int = 3, configs: Optional[Iterable[dict]] = None, save: bool = True,
-         path: str = CACHE, log=print) -> dict:
-    """Time every candidate schedule on one seeded system and cache the fastest."""
+         path: str = CACHE, log=print, nsys: int = 1) -> dict:
+    """Time every candidate schedule on `nsys` seeded systems in one plan (the batch the
+    schedule will serve) and cache the fastest."""
     import blockstructuredsolvers_amd as bss
     from .workloads import spd_system
-    systems = [spd_system(N, n, nrhs=nrhs, seed=1234)]
+    systems = [spd_system(N, n, nrhs=nrhs, seed=1234 + k) for k in range(nsys)]
     results = []
     for cfg in (configs or candidates(n)):
         c = {k: v for k, v in cfg.items() if v is not None}
