@@ -254,9 +254,11 @@ constexpr int factor_min_waves_per_eu() {
 // blocks are then loaded when needed (measured 17% faster at FP32, n = 64; issuing
 // the loads after the potrf instead, in flight through the trsm only, spilled and was
 // 10-25% slower).
+// At n = 96 (FP32, 12 waves, one workgroup per CU) the prefetch registers spill; loading
+// when needed measured 10% faster there.
 template <typename T, int NB, int NW>
 constexpr bool factor_prefetch() {
-  return factor_min_waves_per_eu<T, NB, NW>() == 1;
+  return factor_min_waves_per_eu<T, NB, NW>() == 1 && NB != 96;
 }
 // LAT: the latency variant for levels with at most one chain per CU, where no second
 // workgroup shares the CU: one workgroup per CU, all registers, next blocks prefetched.
