@@ -152,20 +152,22 @@ __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* a
 // 16x16 tiles: right-looking elimination on [A | I] where the diagonal tiles go
 // through diag_chol_inv16 and every panel/trailing/inverse update is an MFMA tile
 // product.  On exit Ubuf's upper tiles hold Uinv = U^{-1}.  Abuf is consumed.
-template <typename T, int NB, int LD, int NW>
+// LA (lookahead): the diagonal step of kk+1 overlaps the elimination of kk (below);
+// without it the diagonal step waits for the whole elimination (the variant with a
+// 128-register cap, where the lookahead's extra live registers spill).
+template <typename T, int NB, int LD, int NW, bool LA>
 __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
   constexpr int TT = NB / 16;
   typedef typename Mfma<T>::acc_t acc_t;
   const int w = wave_id();
-  for (int kk = 0; kk < TT; ++kk) {
-    if (w == 0) {
-      const bool ok = diag_chol_inv16<T>(Abuf + tile_off(kk, kk, LD), LD,
-                                         Ubuf + tile_off(kk, kk, LD), LD);
-      if (!ok && lane_id() == 0) *fail = 1;
-    }
-    __syncthreads();
+  auto diag = [&](int kk) {
+    const bool ok = diag_chol_inv16<T>(Abuf + tile_off(kk, kk, LD), LD,
+                                       Ubuf + tile_off(kk, kk, LD), LD);
+    if (!ok && lane_id() == 0) *fail = 1;
+  };
+  // scaling: U_{kk,j} = Uinv_kk^T A_{kk,j} (j > kk);  Et_{t,kk} <- Et_{t,kk} Uinv_kk (t < kk)
+  auto scale = [&](int kk) {
     const int nrow = TT - 1 - kk;
-    // scaling: U_{kk,j} = Uinv_kk^T A_{kk,j} (j > kk);  Et_{t,kk} <- Et_{t,kk} Uinv_kk (t < kk)
     for (int task = w; task < TT - 1; task += NW) {
       acc_t acc = Mfma<T>::zero();
       if (task < nrow) {
@@ -180,33 +182,70 @@ __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf
         tile_store<T>(Ubuf + tile_off(t, kk, LD), LD, acc);
       }
     }
-    __syncthreads();
-    if (nrow > 0) {
-      // elimination: A_ij -= U_{kk,i}^T U_{kk,j} (kk < i <= j);  Et_{t,i} -= Et_{t,kk} U_{kk,i}
-      const int nA = nrow * (nrow + 1) / 2;
-      const int nE = (kk + 1) * nrow;
-      for (int task = w; task < nA + nE; task += NW) {
-        if (task < nA) {
-          int idx = task, ii = 0;
-          while (idx >= nrow - ii) {
-            idx -= nrow - ii;
-            ++ii;
-          }
-          const int i = kk + 1 + ii, j = i + idx;
-          acc_t acc = tile_load<T>(Abuf + tile_off(i, j, LD), LD);
-          tile_mma<T, true, false, true, 16>(acc, Abuf + tile_off(kk, i, LD), LD,
-                                         Abuf + tile_off(kk, j, LD), LD);
-          tile_store<T>(Abuf + tile_off(i, j, LD), LD, acc);
-        } else {
-          const int e = task - nA;
-          const int t = e % (kk + 1), i = kk + 1 + e / (kk + 1);
-          acc_t acc = (t == kk) ? Mfma<T>::zero() : tile_load<T>(Ubuf + tile_off(t, i, LD), LD);
-          tile_mma<T, false, false, true, 16>(acc, Ubuf + tile_off(t, kk, LD), LD,
-                                          Abuf + tile_off(kk, i, LD), LD);
-          tile_store<T>(Ubuf + tile_off(t, i, LD), LD, acc);
+  };
+  // elimination tasks first, first + stride, ...:
+  //   A_ij -= U_{kk,i}^T U_{kk,j} (kk < i <= j);  Et_{t,i} -= Et_{t,kk} U_{kk,i}
+  // (task 0 is the next diagonal tile)
+  auto eliminate = [&](int kk, int first, int stride) {
+    const int nrow = TT - 1 - kk;
+    const int nA = nrow * (nrow + 1) / 2;
+    const int nE = (kk + 1) * nrow;
+    for (int task = first; task < nA + nE; task += stride) {
+      if (task < nA) {
+        int idx = task, ii = 0;
+        while (idx >= nrow - ii) {
+          idx -= nrow - ii;
+          ++ii;
         }
+        const int i = kk + 1 + ii, j = i + idx;
+        acc_t acc = tile_load<T>(Abuf + tile_off(i, j, LD), LD);
+        tile_mma<T, true, false, true, 16>(acc, Abuf + tile_off(kk, i, LD), LD,
+                                       Abuf + tile_off(kk, j, LD), LD);
+        tile_store<T>(Abuf + tile_off(i, j, LD), LD, acc);
+      } else {
+        const int e = task - nA;
+        const int t = e % (kk + 1), i = kk + 1 + e / (kk + 1);
+        acc_t acc = (t == kk) ? Mfma<T>::zero() : tile_load<T>(Ubuf + tile_off(t, i, LD), LD);
+        tile_mma<T, false, false, true, 16>(acc, Ubuf + tile_off(t, kk, LD), LD,
+                                        Abuf + tile_off(kk, i, LD), LD);
+        tile_store<T>(Ubuf + tile_off(t, i, LD), LD, acc);
       }
+    }
+  };
+  if constexpr (!LA) {
+    for (int kk = 0; kk < TT; ++kk) {
+      if (w == 0) diag(kk);
       __syncthreads();
+      scale(kk);
+      __syncthreads();
+      if (kk < TT - 1) {
+        eliminate(kk, w, NW);
+        __syncthreads();
+      }
+    }
+  } else {
+    // Lookahead: during the elimination of step kk, wave 0 updates the next diagonal
+    // tile (task 0) and factors it at once while the other waves share the remaining
+    // tasks, so the serial diagonal step overlaps the elimination instead of following
+    // it.  LDS accesses of one wave complete in order.
+    if (w == 0) diag(0);
+    __syncthreads();
+    for (int kk = 0; kk < TT; ++kk) {
+      scale(kk);
+      __syncthreads();
+      if (kk < TT - 1) {
+        if (w == 0) {
+          const int i = kk + 1;
+          acc_t acc = tile_load<T>(Abuf + tile_off(i, i, LD), LD);
+          tile_mma<T, true, false, true, 16>(acc, Abuf + tile_off(kk, i, LD), LD,
+                                         Abuf + tile_off(kk, i, LD), LD);
+          tile_store<T>(Abuf + tile_off(i, i, LD), LD, acc);
+          diag(i);
+        } else {
+          eliminate(kk, w, NW - 1);
+        }
+        __syncthreads();
+      }
     }
   }
 }
@@ -337,7 +376,7 @@ __global__ void __launch_bounds__(64 * NW)
     // pivot monitor: wave 0, lane j < NB keeps the diagonal entry A_jj of this block
     float adiag = 0.0f;
     if (p.pivmin && w == 0 && lane_id() < NB) adiag = (float)Abuf[lane_id() * (LD + 1)];
-    potrf_inv<T, NB, LD, NW>(Abuf, Ubuf, &fail);
+    potrf_inv<T, NB, LD, NW, (LAT || factor_min_waves_per_eu<T, NB, NW>() == 1)>(Abuf, Ubuf, &fail);
     // failures and pivot ratios are kept per workgroup and reported once after the
     // chain: a global atomic inside the loop, on a path some lanes skip, would make the
     // memory waits after it conservative (and drain the prefetches)
