@@ -130,13 +130,25 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
         if (j < NB) adiag[h] = (float)tri[Tri::idx(j >> 4, j >> 4) + (j & 15) * (LDT + 1)];
       }
     }
-    // 2. potrf in place: diagonal tiles become Uinv_kk, row panels U_kj
+    // 2. potrf in place: diagonal tiles become Uinv_kk, row panels U_kj.  Lookahead:
+    //    during the trailing update of step kk, wave 0 updates tile (kk+1, kk+1) first
+    //    (task 0) and factors it at once while the other waves share the remaining
+    //    tasks (LDS accesses of one wave complete in order).
+    auto diag = [&](int kk) {
+      const bool ok = diag_chol_inv16<T>(tri + Tri::idx(kk, kk), LDT, tri + Tri::idx(kk, kk), LDT);
+      if (!ok && lane_id() == 0) fail = 1;
+    };
+    auto trailing = [&](int kk, int task) {
+      int ii, jj;
+      upper_tile(task, ii, jj);
+      const int r = kk + 1 + ii, c = kk + 1 + jj;
+      acc_t acc = tile_load<T>(tri + Tri::idx(r, c), LDT);
+      tile_mma<T, true, false, true, 16>(acc, tri + Tri::idx(kk, r), LDT, tri + Tri::idx(kk, c), LDT);
+      tile_store<T>(tri + Tri::idx(r, c), LDT, acc);
+    };
+    if (w == 0) diag(0);
+    __syncthreads();
     for (int kk = 0; kk < TT; ++kk) {
-      if (w == 0) {
-        const bool ok = diag_chol_inv16<T>(tri + Tri::idx(kk, kk), LDT, tri + Tri::idx(kk, kk), LDT);
-        if (!ok && lane_id() == 0) fail = 1;
-      }
-      __syncthreads();
       for (int j = kk + 1 + w; j < TT; j += NW) {
         acc_t acc = Mfma<T>::zero();
         tile_mma<T, true, false, false, 16>(acc, tri + Tri::idx(kk, kk), LDT, tri + Tri::idx(kk, j), LDT);
@@ -145,15 +157,15 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_big_kernel(FactorLevelAr
       __syncthreads();
       const int nrow = TT - 1 - kk;
       const int nA = nrow * (nrow + 1) / 2;
-      for (int task = w; task < nA; task += NW) {
-        int ii, jj;
-        upper_tile(task, ii, jj);
-        const int r = kk + 1 + ii, c = kk + 1 + jj;
-        acc_t acc = tile_load<T>(tri + Tri::idx(r, c), LDT);
-        tile_mma<T, true, false, true, 16>(acc, tri + Tri::idx(kk, r), LDT, tri + Tri::idx(kk, c), LDT);
-        tile_store<T>(tri + Tri::idx(r, c), LDT, acc);
+      if (nrow > 0) {
+        if (w == 0) {
+          trailing(kk, 0);
+          diag(kk + 1);
+        } else {
+          for (int task = w; task < nA; task += NW - 1) trailing(kk, task);
+        }
+        __syncthreads();
       }
-      if (nrow > 0) __syncthreads();
     }
     // reported once after the chain (see chain_factor.hpp)
     if (tid == 0 && fail) {

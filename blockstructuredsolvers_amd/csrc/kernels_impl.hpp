@@ -55,8 +55,10 @@ hipError_t launch_factor(int nb, int waves, const FactorLevelArgs& a, int nchain
     case 96:
       // FP32 factors at n = 96: the four blocks of the LDS kernel fit (154 KB, one
       // workgroup per CU); FP64 ones need the triangle-in-LDS kernel
-      if constexpr (sizeof(T) == 4) return factor_nb<Tin, T, 96, 12>(a, nchains, s);
-      return factor_big_nb<Tin, T, 96, 8>(a, nchains, s);
+      if constexpr (sizeof(T) == 4)
+        return factor_nb<Tin, T, 96, 12>(a, nchains, s);
+      else
+        return factor_big_nb<Tin, T, 96, 8>(a, nchains, s);
     case 128: return factor_big_nb<Tin, T, 128, 8>(a, nchains, s);
     default: return hipErrorInvalidValue;
   }
