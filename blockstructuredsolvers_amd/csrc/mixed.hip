@@ -25,7 +25,6 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 //   x_new = x_old + dx (FP64; written for the workgroup's own blocks),
 //   r = d - (B_{i-1}^T x_{i-1} + A_i x_i + B_i x_{i+1}) in FP64 on MFMA, stored as FP32
 //   for the next correction solve, plus per-column max |r|, |x|, |d|.
-// INIT: x = 0, so r = d with no products (the first pass of a solve).
 // With `mnorm`, the row sums of |M| are accumulated from the operand values the
 // products already hold, so ||M||_inf costs no extra pass over A and B.
 // One wave per 16-row tile; a rolling three-block window of x_new lives in LDS and
@@ -33,7 +32,7 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // Every coupling block is read from HBM once: at block row i, B_i serves both
 // B_i x_{i+1} (row i) and B_i^T x_i (row i+1, carried to the next iteration in
 // registers); the second read of B_i follows the first immediately and hits L2.
-template <int NB, bool INIT>
+template <int NB>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
                     double* __restrict__ part, int nrhs, int xo_k, int dx_k, int xn_k, int ro_k,
@@ -65,22 +64,18 @@ __global__ void __launch_bounds__(4 * NB)
   // workgroup's own blocks, to xnew) by stage_store.  The wait counters are in order,
   // so the x loads are issued before the block's operand loads, and the wait for them
   // leaves the operands in flight; every lane loads (clamped indices), as a predicated
-  // load would make every later wait conservative.  Past the initial pass x_old and
-  // dx are always given.
+  // load would make every later wait conservative.  x_old and dx are always given
+  // (the first pass of a solve is init_pass_kernel).
   constexpr int SE = (NB * 16 + NT - 1) / NT;  // up to 16 right-hand sides
   auto stage_load = [&](int j, double (&v)[SE]) {
     const bool inside = j >= 0 && j < N;
     const long long jc = j < 0 ? 0 : (j >= N ? N - 1 : j);
 #pragma unroll
     for (int u = 0; u < SE; ++u) {
-      if constexpr (INIT) {
-        v[u] = 0.0;
-      } else {
-        const int e = tid + u * NT;
-        const long long o = jc * ND + (e < (int)ND ? e : 0);
-        const double x = xo[o] + (double)dxs[o];
-        v[u] = inside ? x : 0.0;
-      }
+      const int e = tid + u * NT;
+      const long long o = jc * ND + (e < (int)ND ? e : 0);
+      const double x = xo[o] + (double)dxs[o];
+      v[u] = inside ? x : 0.0;
     }
   };
   auto stage_store = [&](int j, const double (&v)[SE]) {
@@ -109,7 +104,7 @@ __global__ void __launch_bounds__(4 * NB)
   __syncthreads();  // window zeroed
   {
     double aBt[KQ];
-    const bool has = !INIT && b0 > 0;
+    const bool has = b0 > 0;
     if (has) {
 #pragma unroll
       for (int q = 0; q < KQ; ++q)
@@ -167,7 +162,7 @@ __global__ void __launch_bounds__(4 * NB)
       const double x = ds[(long long)blk * ND + dc * NB + row];
       acc[v] = (lr < nrhs) ? x : 0.0;
     }
-    if (!INIT) load_chunk(0);
+    load_chunk(0);
 #pragma unroll
     for (int v = 0; v < 4; ++v) dmax = nmax(dmax, fabs(acc[v]));
     __syncthreads();  // previous block's window reads are done
@@ -178,7 +173,7 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       xmax = nmax(xmax, fabs(X0[lr * LDV + 16 * w + Mfma<double>::row(lk, v)]));
-    if (!INIT) {
+    {
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[v] -= carry[v];
       acc_t cn = Mfma<double>::zero();
@@ -285,14 +280,58 @@ __global__ void __launch_bounds__(kRedThreads)
 // 16 blocks each; 24 or 64 blocks were 30% slower)
 int blocks_per_item(int N) { return std::max(32, (N + 255) / 256); }
 
+// The first pass of a solve (x = 0, so r = d): d into the FP32 vector, zeros into the
+// FP64 iterate, and the item's per-column max |d_j| (as both the residual and the
+// right-hand-side maxima; the iterate maximum is 0).  Elementwise, with no block
+// window: one workgroup per residual work item, kInitThreads threads.
+constexpr int kInitThreads = 256;
+
+__global__ void __launch_bounds__(kInitThreads)
+    init_pass_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
+                     double* __restrict__ part, int n, int nrhs, int xn_k, int ro_k) {
+  __shared__ double red[kInitThreads / 64];
+  const ResItem it = items[blockIdx.x];
+  const ResSys& S = systems[it.sys];
+  const long long ND = (long long)n * nrhs;
+  const auto ds = gptr(S.d);
+  const auto xn = gptr(S.x[xn_k]);
+  const auto ro = gptr(S.v[ro_k]);
+  const int tid = threadIdx.x, w = wave_id(), l = lane_id();
+  const int per = kInitThreads / n;  // blocks per sweep of the workgroup (n <= 128)
+  double* out = part + (long long)blockIdx.x * kParts;
+  for (int c = 0; c < nrhs; ++c) {
+    double m = 0.0;
+    for (long long j = it.b0 + tid / n; tid < per * n && j < it.b1; j += per) {
+      const long long e = j * ND + (long long)c * n + tid % n;
+      const double v = ds[e];
+      ro[e] = (float)v;
+      xn[e] = 0.0;
+      m = nmax(m, fabs(v));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = nmax(m, __shfl_xor(m, o));
+    if (l == 0) red[w] = m;
+    __syncthreads();
+    if (tid == 0) {
+      double t = red[0];
+      for (int k = 1; k < kInitThreads / 64; ++k) t = nmax(t, red[k]);
+      out[c] = t;       // max |r_c| (r = d)
+      out[16 + c] = 0;  // max |x_c|
+      out[32 + c] = t;  // max |d_c|
+    }
+    __syncthreads();
+  }
+  if (tid == 0) out[kMnormSlot] = 0.0;
+}
+
 template <int NB>
 hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, int xn_k,
                            int ro_k, int want_mnorm, hipStream_t s) {
   if (init)
-    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nitems), dim3(4 * NB), 0, s, m.items,
-                       m.sys_dev, m.part, m.nrhs, -1, -1, xn_k, ro_k, 0);
+    hipLaunchKernelGGL(init_pass_kernel, dim3(m.nitems), dim3(kInitThreads), 0, s, m.items,
+                       m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k);
   else
-    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
+    hipLaunchKernelGGL((residual_kernel<NB>), dim3(m.nitems), dim3(4 * NB), 0, s,
                        m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   return hipGetLastError();
 }
