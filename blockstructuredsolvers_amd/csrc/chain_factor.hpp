@@ -61,12 +61,14 @@ __device__ inline float read_lane<float>(float x, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
 }
 
-// every lane takes the value of lane J of its 16-lane row (DPP row_newbcast)
+// every lane takes the value of lane J of its 16-lane row (DPP row_newbcast; a
+// broadcast never reads outside the row, so bound_ctrl changes nothing).  The FP64 form
+// is one 64-bit DPP move (gfx950 has row_newbcast for 64-bit operands), not two 32-bit
+// halves that each need a zeroed destination first.
 template <int J>
 __device__ inline double row_bcast(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x150 + J, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x150 + J, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
+  return __longlong_as_double(
+      __builtin_amdgcn_mov_dpp(__double_as_longlong(x), 0x150 + J, 0xf, 0xf, true));
 }
 template <int J>
 __device__ inline float row_bcast(float x) {
@@ -103,21 +105,30 @@ __device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&
   a[J] = ujc;
   e[J] = ejc;
   T s_next = T(0);
+  // U[J][r] = A[J][r] s = A[r][J] s (the trailing matrix stays symmetric): lane J's a[r].
+  // FP32 folds the broadcast into the multiply by s (v_mul_f32 takes a DPP operand, the
+  // FP64 multiply does not): 3 VALU ops per row.  FP64 moves s into the pivot-row
+  // factors once per step instead, so a row costs one 64-bit DPP move and two FMAs.
+  [[maybe_unused]] const T nu = -(ujc * s), ne = -(ejc * s);
+  auto update = [&](int r) {
+    if constexpr (sizeof(T) == 4) {
+      const T ujr = row_bcast<J>(a[r]) * s;
+      a[r] -= ujr * ujc;
+      e[r] -= ujr * ejc;
+    } else {
+      const T m = row_bcast<J>(a[r]);
+      e[r] = fma(m, ne, e[r]);
+      a[r] = fma(m, nu, a[r]);
+    }
+  };
   if constexpr (J < 15) {
-    // U[J][r] = A[J][r] s = A[r][J] s (the trailing matrix stays symmetric): lane J's a[r]
-    const T uj1 = row_bcast<J>(a[J + 1]) * s;
-    a[J + 1] -= uj1 * ujc;
-    e[J + 1] -= uj1 * ejc;
+    update(J + 1);
     const T p = read_lane(a[J + 1], J + 1);  // pivot A[J+1][J+1]
     ok = ok && (p > T(0));
     s_next = rsqrt_nr(p);
   }
 #pragma unroll
-  for (int r = J + 2; r < 16; ++r) {
-    const T ujr = row_bcast<J>(a[r]) * s;
-    a[r] -= ujr * ujc;
-    e[r] -= ujr * ejc;
-  }
+  for (int r = J + 2; r < 16; ++r) update(r);
   // E is consumed only by the final store; without this the compiler defers all of
   // its updates to the end and keeps every multiplier of every step live
 #pragma unroll
