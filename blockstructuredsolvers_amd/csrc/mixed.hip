@@ -32,7 +32,11 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // Every coupling block is read from HBM once: at block row i, B_i serves both
 // B_i x_{i+1} (row i) and B_i^T x_i (row i+1, carried to the next iteration in
 // registers); the second read of B_i follows the first immediately and hits L2.
-template <int NB>
+// V1 (one right-hand side): the products are FP64 FMAs on the same operand registers
+// instead of MFMA.  A 16x16x4 FP64 MFMA issues every 64 cycles on this GPU, the same
+// FLOP rate as the vector FMAs, but with one column it would compute 16 columns to keep
+// one; each lane then sums its 16 k-slots and the four lanes of a row combine.
+template <int NB, bool V1>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
                     double* __restrict__ part, int nrhs, int xo_k, int dx_k, int xn_k, int ro_k,
@@ -98,8 +102,15 @@ __global__ void __launch_bounds__(4 * NB)
   // k index of operand slot q: 4 consecutive k per lane (contiguous loads along k)
   auto kof = [&](int q) { return (q >> 2) * 16 + 4 * lk + (q & 3); };
 
-  // carry: B_{i-1}^T x_{i-1} for the current row (and its |.| row sums for ||M||_inf)
+  // sum over the four lanes (lk) that share row 16w + lr
+  auto row_sum = [&](double v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+  };
+  // carry: B_{i-1}^T x_{i-1} for the current row (and its |.| row sums for ||M||_inf);
+  // V1 keeps it as one value per lane (row 16w + lr)
   acc_t carry = Mfma<double>::zero();
+  [[maybe_unused]] double carry1 = 0.0;
   double rs_carry = 0.0;
   __syncthreads();  // window zeroed
   {
@@ -115,9 +126,15 @@ __global__ void __launch_bounds__(4 * NB)
     __syncthreads();
     if (has) {
       const double* Xm = X[(((b0 - 1) % 3) + 3) % 3];
+      if constexpr (V1) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) t = fma(aBt[q], Xm[kof(q)], t);
+        carry1 = row_sum(t);
+      }
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
-        carry = Mfma<double>::mma(aBt[q], Xm[lr * LDV + kof(q)], carry);
+        if constexpr (!V1) carry = Mfma<double>::mma(aBt[q], Xm[lr * LDV + kof(q)], carry);
         if (mnorm) rs_carry += fabs(aBt[q]);
       }
     }
@@ -155,28 +172,45 @@ __global__ void __launch_bounds__(4 * NB)
     };
     // d_blk, like x, ahead of the operands
     acc_t acc;
-    const int dc = lr < nrhs ? lr : 0;
+    [[maybe_unused]] double d1 = 0.0;
+    if constexpr (V1) {
+      d1 = ds[(long long)blk * ND + 16 * w + lr];
+    } else {
+      const int dc = lr < nrhs ? lr : 0;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = 16 * w + Mfma<double>::row(lk, v);
-      const double x = ds[(long long)blk * ND + dc * NB + row];
-      acc[v] = (lr < nrhs) ? x : 0.0;
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * w + Mfma<double>::row(lk, v);
+        const double x = ds[(long long)blk * ND + dc * NB + row];
+        acc[v] = (lr < nrhs) ? x : 0.0;
+      }
     }
     load_chunk(0);
+    if constexpr (V1) {
+      dmax = nmax(dmax, fabs(d1));
+    } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) dmax = nmax(dmax, fabs(acc[v]));
+      for (int v = 0; v < 4; ++v) dmax = nmax(dmax, fabs(acc[v]));
+    }
     __syncthreads();  // previous block's window reads are done
     stage_store(blk + 1, xs);
     __syncthreads();
     const double* X0 = X[blk % 3];
     const double* Xp = X[(blk + 1) % 3];
+    if constexpr (V1) {
+      xmax = nmax(xmax, fabs(X0[16 * w + lr]));
+    } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      xmax = nmax(xmax, fabs(X0[lr * LDV + 16 * w + Mfma<double>::row(lk, v)]));
+      for (int v = 0; v < 4; ++v)
+        xmax = nmax(xmax, fabs(X0[lr * LDV + 16 * w + Mfma<double>::row(lk, v)]));
+    }
     {
+      if constexpr (!V1) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] -= carry[v];
+        for (int v = 0; v < 4; ++v) acc[v] -= carry[v];
+      }
       acc_t cn = Mfma<double>::zero();
+      // V1 partial sums: A x and B x in two chains, B^T x (the next row's carry)
+      [[maybe_unused]] double sa = 0.0, sb = 0.0, sn = 0.0;
       double rs = rs_carry, rsn = 0.0;
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
@@ -184,16 +218,28 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
         for (int qq = 0; qq < KC; ++qq) {
           const int k = kof(ch * KC + qq);
-          acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
-          acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
-          cn = Mfma<double>::mma(aBn[qq], X0[lr * LDV + k], cn);
+          if constexpr (V1) {
+            const double x0 = X0[k];
+            sa = fma(aA[qq], x0, sa);
+            sb = fma(aB[qq], Xp[k], sb);
+            sn = fma(aBn[qq], x0, sn);
+          } else {
+            acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
+            acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
+            cn = Mfma<double>::mma(aBn[qq], X0[lr * LDV + k], cn);
+          }
           if (mnorm) {
             rs += fabs(aA[qq]) + fabs(aB[qq]);
             rsn += fabs(aBn[qq]);
           }
         }
       }
-      carry = cn;
+      if constexpr (V1) {
+        d1 = d1 - carry1 - row_sum(sa + sb);
+        carry1 = row_sum(sn);
+      } else {
+        carry = cn;
+      }
       rs_carry = rsn;
       if (mnorm) {
         // the four lanes sharing lr hold the four k-slices of row 16w + lr
@@ -202,13 +248,26 @@ __global__ void __launch_bounds__(4 * NB)
         mmax = nmax(mmax, rs);
       }
     }
-    if (lr < nrhs) {
+    if constexpr (V1) {
+      if (lk == 0) ro[(long long)blk * ND + 16 * w + lr] = (float)d1;
+      rmax = nmax(rmax, fabs(d1));
+    } else if (lr < nrhs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         ro[(long long)blk * ND + lr * NB + 16 * w + Mfma<double>::row(lk, v)] = (float)acc[v];
         rmax = nmax(rmax, fabs(acc[v]));
       }
     }
+  }
+  if constexpr (V1) {
+    // lanes hold rows of column 0: fold the wave into column slot 0
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      rmax = nmax(rmax, __shfl_xor(rmax, o));
+      xmax = nmax(xmax, __shfl_xor(xmax, o));
+      dmax = nmax(dmax, __shfl_xor(dmax, o));
+    }
+    if (lr != 0) rmax = xmax = dmax = 0.0;
   }
   red[0][4 * w + lk][lr] = rmax;
   red[1][4 * w + lk][lr] = xmax;
@@ -330,8 +389,11 @@ hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, i
   if (init)
     hipLaunchKernelGGL(init_pass_kernel, dim3(m.nitems), dim3(kInitThreads), 0, s, m.items,
                        m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k);
+  else if (m.nrhs == 1)
+    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nitems), dim3(4 * NB), 0, s,
+                       m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   else
-    hipLaunchKernelGGL((residual_kernel<NB>), dim3(m.nitems), dim3(4 * NB), 0, s,
+    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
                        m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   return hipGetLastError();
 }
