@@ -140,11 +140,21 @@ __global__ void __launch_bounds__(4 * NB)
     }
   }
 
-  // the k range in NCH chunks of KC operand slots (two chunks from n = 64 on), which
-  // bounds the operand registers and measured faster at n = 64
-  constexpr int KC = (KQ <= 8 || KQ % 8 != 0) ? KQ : KQ / 2;
+  // the k range in NCH chunks of KC operand slots (two chunks from n = 64 on with MFMA),
+  // which bounds the operand registers and measured faster at n = 64; V1 up to n = 64
+  // holds fewer registers and takes one chunk (at n = 128 one chunk would spill)
+  constexpr int KC = ((V1 && NB <= 64) || KQ <= 8 || KQ % 8 != 0) ? KQ : KQ / 2;
   constexpr int NCH = KQ / KC;
   static_assert(KC * NCH == KQ && KC % 4 == 0, "operand chunks");
+  // PF: A of the next block row is loaded one row ahead, after this row's other loads,
+  // so that it is still in flight while this row waits for them (V1 at n <= 64, where
+  // the registers allow it; the last row of an item re-reads its own block from L2)
+  constexpr bool PF = V1 && NB <= 64;
+  double pA[KQ];
+  if constexpr (PF) {
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) pA[q] = As[b0 * NN + (long long)kof(q) * NB + 16 * w + lr];
+  }
   for (int blk = b0; blk < b1; ++blk) {
     // operands of this block row, issued first; B_blk twice (row blk, and transposed
     // for row blk + 1 when that row is ours)
@@ -160,7 +170,8 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
       for (int qq = 0; qq < KC; ++qq) {
         const int k = kof(ch * KC + qq);
-        aA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
+        if constexpr (PF) aA[qq] = pA[ch * KC + qq];
+        else aA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
         const double b = Bb[(long long)k * NB + 16 * w + lr];
         aB[qq] = next ? b : 0.0;
       }
@@ -185,6 +196,11 @@ __global__ void __launch_bounds__(4 * NB)
       }
     }
     load_chunk(0);
+    if constexpr (PF) {
+      const int j = blk + 1 < b1 ? blk + 1 : blk;
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) pA[q] = As[j * NN + (long long)kof(q) * NB + 16 * w + lr];
+    }
     if constexpr (V1) {
       dmax = nmax(dmax, fabs(d1));
     } else {
