@@ -117,6 +117,24 @@ struct XOps {
       }
     }
   }
+  // One right-hand side: (op(X) y)_i for row i = lane % 16 over the slices [s_lo, s_hi),
+  // y[k] at yb[k] (LDS), in every lane of the row.  Vector FMAs over the lane's own k
+  // slots, then a sum over the four lanes that share the row.
+  __device__ inline T dot(const T* __restrict__ yb, int s_lo = 0, int s_hi = NB / 16) const {
+    const int lk = lane_id() >> 4;
+    T t0 = T(0), t1 = T(0);
+#pragma unroll
+    for (int s = 0; s < NB / 16; ++s) {
+      if (s < s_lo || s >= s_hi) continue;
+      t0 = fma(a[s][0], yb[16 * s + 4 * lk + 0], t0);
+      t1 = fma(a[s][1], yb[16 * s + 4 * lk + 1], t1);
+      t0 = fma(a[s][2], yb[16 * s + 4 * lk + 2], t0);
+      t1 = fma(a[s][3], yb[16 * s + 4 * lk + 3], t1);
+    }
+    T t = t0 + t1;
+    t += __shfl_xor(t, 16);
+    return t + __shfl_xor(t, 32);
+  }
 };
 
 template <typename T, int NB>
@@ -219,6 +237,92 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
   if (sepL >= 0) store_rhs_tile<T, NB>(gptr((T*)S.rL) + sepL * ND, w, nrhs, accL);
 }
 
+// The forward sweep for one right-hand side (nrhs = 1).  Same schedule, operand loads
+// and hand-off as chain_forward_kernel; the products are vector FMAs (XOps::dot), since
+// an MFMA would compute 16 columns to keep one.  Lane (lr, lk) of wave w holds element
+// 16w + lr of a block's vector; the LDS vectors hold only column 0.
+template <typename T, int NB>
+__global__ void __launch_bounds__(4 * NB) chain_forward1_kernel(SolveLevelArgs p) {
+  constexpr int TT = NB / 16;
+  constexpr long long NN = (long long)NB * NB;
+  __shared__ T Y[NB];
+  __shared__ T R[NB];
+
+  const ChainRec cd = p.chains[blockIdx.x];
+  const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
+  const LevelSys S = p.sys[cd.slot];
+  const auto U = gptr((const T*)S.U);
+  const auto C = gptr((const T*)S.C);
+  const auto W = gptr((const T*)S.W);
+  const auto d = gptr((T*)S.d);
+  const int w = wave_id();
+  const int l = lane_id(), lr = l & 15, lk = l >> 4;
+  const int row = 16 * w + lr;
+  const bool assemble = S.fused && S.dp;
+  struct RhsParts {
+    T a, r, l;
+  };
+  auto rhs_parts = [&](long long q) -> RhsParts {
+    RhsParts t{T(0), T(0), T(0)};
+    if (!assemble) {
+      t.a = d[q * NB + row];
+    } else {
+      t.a = gptr((const T*)S.dp)[(long long)gptr(S.pp)[q] * NB + row];
+      t.r = gptr((const T*)S.rRp)[q * NB + row];
+      t.l = gptr((const T*)S.rLp)[q * NB + row];
+    }
+    return t;
+  };
+  auto rhs_combine = [&](const RhsParts& t) -> T { return assemble ? t.a - t.r - t.l : t.a; };
+  if (assemble && sepR >= 0) {
+    const long long q = start + len;
+    const T v = rhs_combine(rhs_parts(q));
+    if (lk == 0) d[q * NB + row] = v;
+  }
+  if (assemble && sepL >= 0 && start == 1) {
+    const T v = rhs_combine(rhs_parts(0));
+    if (lk == 0) d[row] = v;
+  }
+  if (len <= 0) return;
+
+  XOps<T, NB, true> oc, ou, ow;  // C_{i-1} panel, Uinv column tiles t <= w, W_i panel
+  ou.load(U + (long long)start * NN + 16 * w * NB, NB, 0, w + 1);
+  if (sepL >= 0) ow.load(W + (long long)start * NN + 16 * w * NB, NB);
+  T accL = T(0);
+  RhsParts rnext = rhs_parts(start);
+  for (int i = 0; i < len; ++i) {
+    const long long blk = start + i;
+    T v = rhs_combine(rnext);
+    if (i + 1 < len) rnext = rhs_parts(blk + 1);
+    if (i > 0) v -= oc.dot(Y);
+    if (lk == 0) R[row] = v;
+    if (i + 1 < len) oc.load(C + blk * NN + 16 * w * NB, NB);
+    if (sepL >= 0 && i > 0) ow.load(W + blk * NN + 16 * w * NB, NB);
+    __syncthreads();
+    v = ou.dot(R, 0, w + 1);  // Uinv_blk^T r
+    if (lk == 0) {
+      d[blk * NB + row] = v;
+      Y[row] = v;
+    }
+    if (i + 1 < len) ou.load(U + (blk + 1) * NN + 16 * w * NB, NB, 0, w + 1);
+    __syncthreads();
+    if (sepL >= 0) accL += ow.dot(Y);
+  }
+  if (sepR >= 0) {
+    // C_last^T y_last, straight from HBM (one use)
+    const auto cb = C + (long long)(start + len - 1) * NN + (long long)row * NB;
+    T t = T(0);
+#pragma unroll
+    for (int s = 0; s < TT; ++s)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) t = fma(cb[16 * s + 4 * lk + q], Y[16 * s + 4 * lk + q], t);
+    t += __shfl_xor(t, 16);
+    t += __shfl_xor(t, 32);
+    if (lk == 0) gptr((T*)S.rR)[(long long)sepR * NB + row] = t;
+  }
+  if (sepL >= 0 && lk == 0) gptr((T*)S.rL)[(long long)sepL * NB + row] = accL;
+}
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p) {
   constexpr int TT = NB / 16;
@@ -285,6 +389,74 @@ __global__ void __launch_bounds__(4 * NB) chain_backward_kernel(SolveLevelArgs p
     ou.template mma<false>(acc, R, LDV, w, TT);  // Uinv_blk t
     store_rhs_tile<T, NB>(d + blk * ND, w, nrhs, acc);
     tile_store<T>(Xn + 16 * w, LDV, acc);
+    if (i > 0) ou.load(U + (blk - 1) * NN + 16 * w, NB, w, TT);
+    __syncthreads();
+  }
+}
+
+// The backward sweep for one right-hand side (layout as chain_forward1_kernel).
+template <typename T, int NB>
+__global__ void __launch_bounds__(4 * NB) chain_backward1_kernel(SolveLevelArgs p) {
+  constexpr int TT = NB / 16;
+  constexpr long long NN = (long long)NB * NB;
+  __shared__ T Xn[NB];
+  __shared__ T XL[NB];
+  __shared__ T R[NB];
+
+  const ChainRec cd = p.chains[blockIdx.x];
+  const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
+  const LevelSys S = p.sys[cd.slot];
+  const auto U = gptr((const T*)S.U);
+  const auto C = gptr((const T*)S.C);
+  const auto W = gptr((const T*)S.W);
+  const auto d = gptr((T*)S.d);
+  const auto xs = gptr((const T*)S.dn);
+  const int w = wave_id();
+  const int l = lane_id(), lr = l & 15, lk = l >> 4;
+  const int row = 16 * w + lr;
+  if (S.fused && sepR >= 0) {
+    const T v = xs[(long long)sepR * NB + row];
+    if (lk == 0) d[(long long)(start + len) * NB + row] = v;
+  }
+  if (S.fused && sepL >= 0 && start == 1) {
+    const T v = xs[(long long)sepL * NB + row];
+    if (lk == 0) d[row] = v;
+  }
+  if (len <= 0) return;
+
+  // the separator solutions (read only when the separator exists)
+  if (threadIdx.x < NB) {
+    const int k = threadIdx.x;
+    if (sepR >= 0) Xn[k] = xs[(long long)sepR * NB + k];
+    if (sepL >= 0) XL[k] = xs[(long long)sepL * NB + k];
+  }
+  __syncthreads();
+
+  XOps<T, NB, false> oc, ow, ou;  // rows 16w.. of C_i, W_i, and of Uinv_i (tiles t >= w)
+  {
+    const long long blk = start + len - 1;
+    if (sepR >= 0) oc.load(C + blk * NN + 16 * w, NB);
+    if (sepL >= 0) ow.load(W + blk * NN + 16 * w, NB);
+    ou.load(U + blk * NN + 16 * w, NB, w, TT);
+  }
+  T rnext = d[(long long)(start + len - 1) * NB + row];
+  for (int i = len - 1; i >= 0; --i) {
+    const long long blk = start + i;
+    T v = rnext;
+    if (i > 0) rnext = d[(blk - 1) * NB + row];
+    if (i < len - 1 || sepR >= 0) v -= oc.dot(Xn);
+    if (sepL >= 0) v -= ow.dot(XL);
+    if (lk == 0) R[row] = v;
+    if (i > 0) {
+      oc.load(C + (blk - 1) * NN + 16 * w, NB);
+      if (sepL >= 0) ow.load(W + (blk - 1) * NN + 16 * w, NB);
+    }
+    __syncthreads();
+    v = ou.dot(R, w, TT);  // Uinv_blk t
+    if (lk == 0) {
+      d[blk * NB + row] = v;
+      Xn[row] = v;
+    }
     if (i > 0) ou.load(U + (blk - 1) * NN + 16 * w, NB, w, TT);
     __syncthreads();
   }

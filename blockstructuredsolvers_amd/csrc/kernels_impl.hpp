@@ -70,7 +70,10 @@ hipError_t launch_forward(int nb, const SolveLevelArgs& a, int nchains, hipStrea
   switch (nb) {
 #define BSS_FWD(NB)                                                                          \
   case NB:                                                                                   \
-    hipLaunchKernelGGL((chain_forward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a);  \
+    if (a.nrhs == 1)                                                                         \
+      hipLaunchKernelGGL((chain_forward1_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a); \
+    else                                                                                     \
+      hipLaunchKernelGGL((chain_forward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a); \
     return hipGetLastError();
     BSS_FWD(16) BSS_FWD(32) BSS_FWD(48) BSS_FWD(64) BSS_FWD(96) BSS_FWD(128)
 #undef BSS_FWD
@@ -84,7 +87,10 @@ hipError_t launch_backward(int nb, const SolveLevelArgs& a, int nchains, hipStre
   switch (nb) {
 #define BSS_BWD(NB)                                                                           \
   case NB:                                                                                    \
-    hipLaunchKernelGGL((chain_backward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a);  \
+    if (a.nrhs == 1)                                                                          \
+      hipLaunchKernelGGL((chain_backward1_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a); \
+    else                                                                                      \
+      hipLaunchKernelGGL((chain_backward_kernel<T, NB>), dim3(nchains), dim3(4 * NB), 0, s, a); \
     return hipGetLastError();
     BSS_BWD(16) BSS_BWD(32) BSS_BWD(48) BSS_BWD(64) BSS_BWD(96) BSS_BWD(128)
 #undef BSS_BWD
