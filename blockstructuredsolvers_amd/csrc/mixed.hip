@@ -36,16 +36,23 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // instead of MFMA.  A 16x16x4 FP64 MFMA issues every 64 cycles on this GPU, the same
 // FLOP rate as the vector FMAs, but with one column it would compute 16 columns to keep
 // one; each lane then sums its 16 k-slots and the four lanes of a row combine.
-template <int NB, bool V1>
+// TS (V1 up to n = 64, no ||M|| pass): B_i^T x_i is not read again from L2 as a
+// transposed operand; the products B_i[r][k] x_i[r] of the registers that serve B_i
+// x_{i+1} go to an LDS tile, and the next row sums its columns.
+template <int NB, bool V1, bool TS>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
                     double* __restrict__ part, int nrhs, int xo_k, int dx_k, int xn_k, int ro_k,
-                    int mnorm) {
+                    int mnorm_arg) {
+  static_assert(!TS || (V1 && NB <= 64), "transpose-sum path: one column, n <= 64");
   constexpr int TT = NB / 16, NT = 64 * TT, LDV = NB + 2, KQ = NB / 4;
   constexpr long long NN = (long long)NB * NB;
   typedef Mfma<double>::acc_t acc_t;
   __shared__ double X[3][16 * LDV];
   __shared__ double red[4][4 * TT][16];
+  constexpr int LDP = NB + 1;
+  __shared__ double P[TS ? NB * LDP : 1];  // TS: B_i[r][k] x_i[r] at P[r * LDP + k]
+  const int mnorm = TS ? 0 : mnorm_arg;
   const ResItem it = items[blockIdx.x];
   const ResSys& S = systems[it.sys];
   const int N = S.N;
@@ -175,10 +182,12 @@ __global__ void __launch_bounds__(4 * NB)
         const double b = Bb[(long long)k * NB + 16 * w + lr];
         aB[qq] = next ? b : 0.0;
       }
+      if constexpr (!TS) {
 #pragma unroll
-      for (int qq = 0; qq < KC; ++qq) {
-        const double b = Bb[(long long)(16 * w + lr) * NB + kof(ch * KC + qq)];
-        aBn[qq] = carry_on ? b : 0.0;
+        for (int qq = 0; qq < KC; ++qq) {
+          const double b = Bb[(long long)(16 * w + lr) * NB + kof(ch * KC + qq)];
+          aBn[qq] = carry_on ? b : 0.0;
+        }
       }
     };
     // d_blk, like x, ahead of the operands
@@ -208,6 +217,17 @@ __global__ void __launch_bounds__(4 * NB)
       for (int v = 0; v < 4; ++v) dmax = nmax(dmax, fabs(acc[v]));
     }
     __syncthreads();  // previous block's window reads are done
+    if constexpr (TS) {
+      // this row's carry: column 16w + lr of the previous row's product tile (written
+      // after the previous row's second barrier, rewritten only after this row's)
+      // (the four lanes of a row each sum NB / 4 of the tile's rows)
+      if (blk > b0) {
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < NB / 4; ++r) t += P[((NB / 4) * lk + r) * LDP + 16 * w + lr];
+        carry1 = row_sum(t);
+      }
+    }
     stage_store(blk + 1, xs);
     __syncthreads();
     const double* X0 = X[blk % 3];
@@ -238,7 +258,7 @@ __global__ void __launch_bounds__(4 * NB)
             const double x0 = X0[k];
             sa = fma(aA[qq], x0, sa);
             sb = fma(aB[qq], Xp[k], sb);
-            sn = fma(aBn[qq], x0, sn);
+            if constexpr (!TS) sn = fma(aBn[qq], x0, sn);
           } else {
             acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
             acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
@@ -250,7 +270,14 @@ __global__ void __launch_bounds__(4 * NB)
           }
         }
       }
-      if constexpr (V1) {
+      if constexpr (TS) {
+        d1 = d1 - carry1 - row_sum(sa + sb);
+        if (carry_on) {
+          const double xr = X0[16 * w + lr];
+#pragma unroll
+          for (int q = 0; q < KQ; ++q) P[(16 * w + lr) * LDP + kof(q)] = aB[q] * xr;
+        }
+      } else if constexpr (V1) {
         d1 = d1 - carry1 - row_sum(sa + sb);
         carry1 = row_sum(sn);
       } else {
@@ -405,11 +432,14 @@ hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, i
   if (init)
     hipLaunchKernelGGL(init_pass_kernel, dim3(m.nitems), dim3(kInitThreads), 0, s, m.items,
                        m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k);
+  else if (m.nrhs == 1 && NB <= 64 && !want_mnorm)
+    hipLaunchKernelGGL((residual_kernel<NB, true, NB <= 64>), dim3(m.nitems), dim3(4 * NB), 0,
+                       s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, 0);
   else if (m.nrhs == 1)
-    hipLaunchKernelGGL((residual_kernel<NB, true>), dim3(m.nitems), dim3(4 * NB), 0, s,
+    hipLaunchKernelGGL((residual_kernel<NB, true, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
                        m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   else
-    hipLaunchKernelGGL((residual_kernel<NB, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
+    hipLaunchKernelGGL((residual_kernel<NB, false, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
                        m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
   return hipGetLastError();
 }
