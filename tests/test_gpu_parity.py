@@ -159,7 +159,8 @@ def run_gpu_batch(systems, T=torch.float64, **kw):
     return [data.d_tensor[:, :, :N, s].double().cpu().numpy() for s in range(len(systems))], data
 
 
-@pytest.mark.parametrize("n,nrhs", [(32, 1), (64, 1), (64, 16), (16, 3), (96, 2), (128, 1), (128, 16)])
+@pytest.mark.parametrize("n,nrhs", [(16, 1), (32, 1), (48, 1), (64, 1), (96, 1), (64, 16), (16, 3),
+                                    (96, 2), (128, 1), (128, 16)])
 def test_mixed_refined_backward_error(n, nrhs):
     """FP32 MFMA factorisation + FP64 refinement: normwise backward error <= 1e-12 as
     measured by the FP64 oracle, agreeing with the device's own measurement."""
