@@ -13,7 +13,7 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    # GPU tests are only meaningful on a GPU box; on a CPU-only host they are skipped
+    # GPU tests are only meaningful on a host with a GPU; on a CPU-only host they are skipped
     # unless explicitly selected with -m gpu (which then fails loudly).
     try:
         import torch

@@ -33,7 +33,7 @@ def flat(x):
 @pytest.mark.parametrize("N,n,nrhs", [(1, 5, 1), (2, 3, 2), (9, 4, 1), (50, 8, 3), (20, 32, 2)])
 def test_oracle_vs_dense_cholesky(N, n, nrhs):
     # A_i = G G^T + alpha n I is SPD with B_i ~ N(0,1) only once n*alpha dominates
-    # ||B||; the survey checked alpha = 1 for n >= 16, tiny blocks need a larger shift
+    # ||B||; alpha = 1 suffices for n >= 16 (the blocks are then SPD), tiny blocks need a larger shift
     s = workloads.spd_system(N, n, nrhs=nrhs, seed=N * 100 + n, alpha=1.0 if n >= 16 else 4.0)
     x, info = oracle.solve_system(s.A, s.B, s.d)
     assert info == 0
@@ -137,8 +137,8 @@ def test_reference_batched_restatement_agrees():
 
 
 def test_reference_schedule_hand_table():
-    """find_sequence_upper (gpu_batched.jl:155-162) at the sizes the survey checked by
-    hand; N_list top level = N_use; N_i = (P_i - 1) m_i + P_i; P_{i+1} = N_i."""
+    """find_sequence_upper (gpu_batched.jl:155-162) at sizes worked out by hand from
+    the recursion; N_list top level = N_use; N_i = (P_i - 1) m_i + P_i; P_{i+1} = N_i."""
     table = {
         50: (50, [(50, 6, 8)]),
         100: (100, [(100, 8, 12)]),
