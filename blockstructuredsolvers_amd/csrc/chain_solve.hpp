@@ -131,9 +131,7 @@ struct XOps {
       t0 = fma(a[s][2], yb[16 * s + 4 * lk + 2], t0);
       t1 = fma(a[s][3], yb[16 * s + 4 * lk + 3], t1);
     }
-    T t = t0 + t1;
-    t += __shfl_xor(t, 16);
-    return t + __shfl_xor(t, 32);
+    return row4_sum(t0 + t1);
   }
 };
 
@@ -316,8 +314,7 @@ __global__ void __launch_bounds__(4 * NB) chain_forward1_kernel(SolveLevelArgs p
     for (int s = 0; s < TT; ++s)
 #pragma unroll
       for (int q = 0; q < 4; ++q) t = fma(cb[16 * s + 4 * lk + q], Y[16 * s + 4 * lk + q], t);
-    t += __shfl_xor(t, 16);
-    t += __shfl_xor(t, 32);
+    t = row4_sum(t);
     if (lk == 0) gptr((T*)S.rR)[(long long)sepR * NB + row] = t;
   }
   if (sepL >= 0 && lk == 0) gptr((T*)S.rL)[(long long)sepL * NB + row] = accL;

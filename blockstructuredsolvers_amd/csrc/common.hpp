@@ -49,6 +49,30 @@ struct Mfma<float> {
 };
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
+
+// Sum over the four 16-lane rows of the wave (lanes l, l ^ 16, l ^ 32, l ^ 48), in
+// every lane, as (t0 + t1) + (t2 + t3).  gfx950's row swaps (v_permlane16_swap and
+// v_permlane32_swap) are VALU operations; a shuffle would be two round trips through
+// the LDS crossbar (ds_bpermute) on the critical path of every block step.
+__device__ inline float row4_sum(float t) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t), __float_as_uint(t),
+                                                  false, false);
+  t = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(t), __float_as_uint(t),
+                                                  false, false);
+  return __uint_as_float(q[0]) + __uint_as_float(q[1]);
+}
+__device__ inline double row4_sum(double t) {
+  unsigned lo = (unsigned)__double2loint(t), hi = (unsigned)__double2hiint(t);
+  auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  t = __hiloint2double((int)rh[0], (int)rl[0]) + __hiloint2double((int)rh[1], (int)rl[1]);
+  lo = (unsigned)__double2loint(t);
+  hi = (unsigned)__double2hiint(t);
+  rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)rh[0], (int)rl[0]) + __hiloint2double((int)rh[1], (int)rl[1]);
+}
 // wave index, in a scalar register: branches on it are uniform, so loads under them stay
 // unpredicated and the wait counters can be tracked exactly
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }

@@ -110,10 +110,7 @@ __global__ void __launch_bounds__(4 * NB)
   auto kof = [&](int q) { return (q >> 2) * 16 + 4 * lk + (q & 3); };
 
   // sum over the four lanes (lk) that share row 16w + lr
-  auto row_sum = [&](double v) {
-    v += __shfl_xor(v, 16);
-    return v + __shfl_xor(v, 32);
-  };
+  auto row_sum = [&](double v) { return row4_sum(v); };
   // carry: B_{i-1}^T x_{i-1} for the current row (and its |.| row sums for ||M||_inf);
   // V1 keeps it as one value per lane (row 16w + lr)
   acc_t carry = Mfma<double>::zero();
@@ -286,8 +283,7 @@ __global__ void __launch_bounds__(4 * NB)
       rs_carry = rsn;
       if (mnorm) {
         // the four lanes sharing lr hold the four k-slices of row 16w + lr
-        rs += __shfl_xor(rs, 16);
-        rs += __shfl_xor(rs, 32);
+        rs = row4_sum(rs);
         mmax = nmax(mmax, rs);
       }
     }
