@@ -562,7 +562,7 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
   p->sep64 = false;
   p->refactored = 0;
   if (p->gl.size() < 2) return BSS_OK;
-  HIP_TRY(hipEventSynchronize(p->piv_event));
+  HIP_TRY(bss::wait_event_spin(p->piv_event));
   bool need = false;
   for (int q = 0; q < p->nsys; ++q) {
     float r;

@@ -42,6 +42,7 @@ struct MixedState {
   double* dnorm = nullptr;  // ||d_j||_inf per system and column
   double* berr = nullptr;   // backward error per system and column
   double* host_berr = nullptr;  // pinned mirror of berr
+  hipEvent_t berr_ready = nullptr;  // recorded after each copy of berr to host_berr
   bool mnorm_stale = true;      // ||M|| must be recomputed (new factorisation)
   std::vector<double> last_berr;
 
@@ -55,6 +56,12 @@ struct MixedState {
   void release();
   ~MixedState() { release(); }
 };
+
+// Wait for `ev` by polling it for up to about two milliseconds, then by blocking.  The
+// refinement loop waits for a few bytes once per iteration, with the device idle until
+// the host has queued the next iteration: a blocking wait adds its wake-up time to
+// that idle gap.
+hipError_t wait_event_spin(hipEvent_t ev);
 
 struct MixedSolveHooks {
   void* ctx = nullptr;
