@@ -476,11 +476,12 @@ hipError_t residual_prof(const MixedState& m, const MixedSolveHooks& h, bool ini
 size_t MixedState::bytes(const std::vector<int>& Ns, int n, int nrhs) {
   size_t b = 0, items = 0;
   const size_t S = Ns.size();
+  size_t nd_all = 0;
   for (int N : Ns) {
-    const size_t nd = (size_t)N * n * nrhs;
-    b += 2 * al(nd * 8) + 2 * al(nd * 4);
+    nd_all += (size_t)N * n * nrhs;
     items += (N + blocks_per_item(N) - 1) / blocks_per_item(N);
   }
+  b += 2 * al(nd_all * 8) + 2 * al(nd_all * 4);
   b += al(items * sizeof(ResItem)) + al(S * sizeof(ResSys)) + al(items * kParts * 8);
   b += al(S * 8) + al(S * 16 * 8) + al(S * 16 * 8);
   return b;
@@ -499,14 +500,24 @@ int MixedState::attach(char* base, const std::vector<int>& Ns_, int n_, int nrhs
   };
   sys_host.assign(nsys, ResSys{});
   std::vector<ResItem> it;
+  // each vector set packs the systems like the caller's d (system s at the sum of the
+  // earlier systems' N * n * nrhs), so the final iterate leaves in one copy
+  size_t nd_all = 0;
+  for (int N : Ns) nd_all += (size_t)N * n * nrhs;
+  double* x0 = (double*)take(nd_all * 8);
+  double* x1 = (double*)take(nd_all * 8);
+  float* v0 = (float*)take(nd_all * 4);
+  float* v1 = (float*)take(nd_all * 4);
+  size_t od = 0;
   for (int s = 0; s < nsys; ++s) {
     const int N = Ns[s];
     const size_t nd = (size_t)N * n * nrhs;
     ResSys& r = sys_host[s];
-    r.x[0] = (double*)take(nd * 8);
-    r.x[1] = (double*)take(nd * 8);
-    r.v[0] = (float*)take(nd * 4);
-    r.v[1] = (float*)take(nd * 4);
+    r.x[0] = x0 + od;
+    r.x[1] = x1 + od;
+    r.v[0] = v0 + od;
+    r.v[1] = v1 + od;
+    od += nd;
     r.N = N;
     r.item0 = (int)it.size();
     const int bpw = blocks_per_item(N);
@@ -635,16 +646,13 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
     }
     if ((!nan && worst <= tol) || it >= max_iters) break;
   }
-  // the final iterate into the caller's d (packed like d)
-  size_t od = 0;
-  for (int sy = 0; sy < m.nsys; ++sy) {
-    const size_t nd = (size_t)m.Ns[sy] * m.n * m.nrhs;
-    e = hipMemcpyAsync(d + od, m.sys_host[sy].x[xa], nd * 8, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) {
-      g_mixed_err = hipGetErrorString(e);
-      return -1;
-    }
-    od += nd;
+  // the final iterate into the caller's d (the iterate set is packed like d)
+  size_t nd_all = 0;
+  for (int sy = 0; sy < m.nsys; ++sy) nd_all += (size_t)m.Ns[sy] * m.n * m.nrhs;
+  e = hipMemcpyAsync(d, m.sys_host[0].x[xa], nd_all * 8, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) {
+    g_mixed_err = hipGetErrorString(e);
+    return -1;
   }
   *iters = it;
   return 0;
