@@ -49,11 +49,14 @@ int fail(int code, const char* fmt, ...) {
 
 constexpr int kInfoNone = 0x7f7f7f7f;  // "no failure" sentinel
 
-// Resets per-system status words.  A kernel rather than hipMemsetAsync, so the
-// factorisation can be captured into a hipGraph.
-__global__ void fill_kernel(int* __restrict__ dst, int n, int value) {
+// Resets the per-system status words (info, and the pivot monitor when `piv` is set) in
+// one launch.  A kernel rather than hipMemsetAsync, so the factorisation can be captured
+// into a hipGraph.
+__global__ void reset_status_kernel(int* __restrict__ info, int ninfo, int vinfo,
+                                    int* __restrict__ piv, int npiv, int vpiv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = value;
+  if (i < ninfo) info[i] = vinfo;
+  if (piv && i < npiv) piv[i] = vpiv;
 }
 constexpr int kPivNone = 0x7f800000;  // float +inf bits: no pivot seen
 constexpr size_t kNone = ~size_t(0);   // buffer not allocated
@@ -136,14 +139,14 @@ struct bss_plan {
   std::vector<TabOff> toff;
   size_t info_off = kNone;
   // separator re-factorisation (mixed mode)
-  size_t piv_off = kNone, widen_off = kNone, narrow_off = kNone;
+  size_t widen_off = kNone, narrow_off = kNone;
   unsigned* dpiv = nullptr;
   bss::ConvRec<float, double>* dwiden = nullptr;
   bss::ConvRec<double, float>* dnarrow = nullptr;
   int nconv = 0;
   bool pivots_pending = false;  // factorized, pivots not yet inspected
-  unsigned* host_piv = nullptr;  // pinned: [0, nsys) pivot ratios, [nsys, 2 nsys) FP32
-                                 // separator info words, copied after each factorize
+  unsigned* host_piv = nullptr;  // pinned: [0, nsys) FP32 separator info words,
+                                 // [nsys, 2 nsys) pivot ratios, copied after each factorize
   hipEvent_t piv_event = nullptr;
   bool sep64 = false;           // separator levels currently factored in FP64
   hipGraphExec_t g_solve64[2] = {nullptr, nullptr};
@@ -306,10 +309,11 @@ int layout(bss_plan* p) {
       if (mixed) p->toff[l].tab64[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
     }
   }
-  // info words: [0, nsys) level 0 and FP64 levels, [nsys, 2 nsys) FP32 separator levels
-  p->info_off = bump.take(sizeof(int) * 2 * p->nsys);
+  // status words: [0, nsys) info of level 0 and FP64 levels, [nsys, 2 nsys) info of the
+  // FP32 separator levels; mixed mode adds [2 nsys, 3 nsys) pivot monitor, so that the
+  // host reads separator info and pivots in one copy
+  p->info_off = bump.take(sizeof(int) * (mixed ? 3 : 2) * p->nsys);
   if (mixed) {
-    p->piv_off = bump.take(sizeof(unsigned) * p->nsys);
     p->widen_off = bump.take(sizeof(bss::ConvRec<float, double>) * p->nsys);
     p->narrow_off = bump.take(sizeof(bss::ConvRec<double, float>) * p->nsys);
   }
@@ -460,7 +464,7 @@ int build(bss_plan* p) {
   }
   if (p->mode == BSS_MODE_MIXED) {
     // level-1 couplings FP32 -> FP64 and level-1 vectors FP64 -> FP32, per system
-    p->dpiv = (unsigned*)(p->ws + p->piv_off);
+    p->dpiv = (unsigned*)(p->dinfo + 2 * p->nsys);
     p->dwiden = (bss::ConvRec<float, double>*)(p->ws + p->widen_off);
     p->dnarrow = (bss::ConvRec<double, float>*)(p->ws + p->narrow_off);
     std::vector<bss::ConvRec<float, double>> wr;
@@ -493,11 +497,9 @@ template <typename Tin, typename T>
 int factor_levels(bss_plan* p, hipStream_t s) {
   const int nb = p->n;
   const bool mixed = p->mode == BSS_MODE_MIXED;
-  hipLaunchKernelGGL(fill_kernel, dim3((2 * p->nsys + 255) / 256), dim3(256), 0, s, p->dinfo,
-                     2 * p->nsys, kInfoNone);
-  if (mixed)
-    hipLaunchKernelGGL(fill_kernel, dim3((p->nsys + 255) / 256), dim3(256), 0, s, (int*)p->dpiv,
-                       p->nsys, kPivNone);
+  hipLaunchKernelGGL(reset_status_kernel, dim3((2 * p->nsys + 255) / 256), dim3(256), 0, s,
+                     p->dinfo, 2 * p->nsys, kInfoNone, mixed ? (int*)p->dpiv : nullptr, p->nsys,
+                     kPivNone);
   HIP_TRY(hipGetLastError());
   for (size_t l = 0; l < p->gl.size(); ++l) {
     GLevel& G = p->gl[l];
@@ -566,8 +568,8 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
   bool need = false;
   for (int q = 0; q < p->nsys; ++q) {
     float r;
-    memcpy(&r, &p->host_piv[q], sizeof(r));
-    if (!(r >= (float)p->pivot_tol) || (int)p->host_piv[p->nsys + q] != kInfoNone) need = true;
+    memcpy(&r, &p->host_piv[p->nsys + q], sizeof(r));
+    if (!(r >= (float)p->pivot_tol) || (int)p->host_piv[q] != kInfoNone) need = true;
   }
   if (!need) return BSS_OK;
   if (int e = refactor_separators64(p, s)) return e;
@@ -898,10 +900,8 @@ int bss_factorize(bss_plan_t p, void* stream) {
   if (p->mode == BSS_MODE_MIXED) {
     rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, float>(p, c); });
     if (rc == BSS_OK && p->gl.size() > 1) {
-      // pivot ratios and FP32 separator info to pinned memory; read at the first solve
-      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dpiv, sizeof(unsigned) * p->nsys,
-                             hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipMemcpyAsync(p->host_piv + p->nsys, p->dinfo + p->nsys, sizeof(int) * p->nsys,
+      // FP32 separator info and pivot ratios to pinned memory; read at the first solve
+      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
                              hipMemcpyDeviceToHost, s));
       HIP_TRY(hipEventRecord(p->piv_event, s));
     }
