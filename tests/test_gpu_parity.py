@@ -376,3 +376,17 @@ def test_seq_solver_matches_oracle():
     torch.cuda.synchronize()
     check_f64(sysm, data.d_tensor.cpu().numpy())
     assert data.info()[0] == 0
+
+
+def test_mixed_first_pass_check_depends_on_tolerance():
+    """The refinement loop skips the host's backward-error check after the first FP32
+    pass only when the tolerance lies below what one FP32 solve can reach (1e-9): a
+    loose tolerance (1e-5) stops after one correction, the default (1e-12) needs at
+    least two, and both meet their tolerance by the oracle's measure."""
+    sysm = workloads.spd_system(400, 64, nrhs=1, seed=901)
+    x, data = run_gpu(sysm, mode="mixed", tol=1e-5)
+    assert data.iterations == 1
+    assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-5
+    x, data = run_gpu(sysm, mode="mixed")
+    assert data.iterations >= 2
+    assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
