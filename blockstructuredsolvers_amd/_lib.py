@@ -54,6 +54,7 @@ SIGNATURES = {
     "bss_build_smoother": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP], _I),
     "bss_factorize": ([_VP, _VP], _I),
     "bss_solve": ([_VP, _VP], _I),
+    "bss_factorize_solve": ([_VP, _VP], _I),
     "bss_info": ([_VP, ctypes.POINTER(_I)], _I),
     "bss_last_backward_error": ([_VP, ctypes.POINTER(_D)], _I),
     "bss_sync": ([_VP], _I),

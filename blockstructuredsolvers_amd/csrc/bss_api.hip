@@ -154,6 +154,11 @@ struct bss_plan {
   bool use_graph = false;
   hipStream_t cap_stream = nullptr;  // capture happens here, replay on the caller's stream
   hipGraphExec_t g_factor = nullptr, g_solve[2] = {nullptr, nullptr};
+  // bss_factorize_solve: the first solve's level-0 work runs on side_stream while the
+  // separator levels are factored; l0_done tells the next solve32 to skip it
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_l0_factored = nullptr, ev_l0_solved = nullptr;
+  bool l0_done = false;
   void drop_graphs() {
     for (hipGraphExec_t* g : {&g_factor, &g_solve[0], &g_solve[1], &g_solve64[0], &g_solve64[1]}) {
       if (*g) (void)hipGraphExecDestroy(*g);
@@ -165,6 +170,9 @@ struct bss_plan {
     if (piv_event) (void)hipEventDestroy(piv_event);
     if (host_piv) (void)hipHostFree(host_piv);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (ev_l0_factored) (void)hipEventDestroy(ev_l0_factored);
+    if (ev_l0_solved) (void)hipEventDestroy(ev_l0_solved);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
   }
 };
@@ -494,14 +502,16 @@ int build(bss_plan* p) {
 }
 
 template <typename Tin, typename T>
-int factor_levels(bss_plan* p, hipStream_t s) {
+int factor_levels(bss_plan* p, hipStream_t s, size_t l_begin = 0, size_t l_end = ~size_t(0)) {
   const int nb = p->n;
   const bool mixed = p->mode == BSS_MODE_MIXED;
-  hipLaunchKernelGGL(reset_status_kernel, dim3((2 * p->nsys + 255) / 256), dim3(256), 0, s,
-                     p->dinfo, 2 * p->nsys, kInfoNone, mixed ? (int*)p->dpiv : nullptr, p->nsys,
-                     kPivNone);
-  HIP_TRY(hipGetLastError());
-  for (size_t l = 0; l < p->gl.size(); ++l) {
+  if (l_begin == 0) {
+    hipLaunchKernelGGL(reset_status_kernel, dim3((2 * p->nsys + 255) / 256), dim3(256), 0, s,
+                       p->dinfo, 2 * p->nsys, kInfoNone, mixed ? (int*)p->dpiv : nullptr,
+                       p->nsys, kPivNone);
+    HIP_TRY(hipGetLastError());
+  }
+  for (size_t l = l_begin; l < std::min(l_end, p->gl.size()); ++l) {
     GLevel& G = p->gl[l];
     // mixed mode: FP32 separator levels report into their own info words and feed the
     // pivot monitor that decides on an FP64 re-factorisation
@@ -581,10 +591,10 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
 // Solve in place in the level-0 vectors of set k (plain: the caller's d; mixed: the
 // FP32 ping-pong vector k).
 template <typename T>
-int solve_levels(bss_plan* p, int k, hipStream_t s) {
+int solve_levels(bss_plan* p, int k, hipStream_t s, size_t l_begin = 0) {
   const int nb = p->n;
   const size_t L = p->gl.size();
-  for (size_t l = 0; l < L; ++l) {
+  for (size_t l = l_begin; l < L; ++l) {
     GLevel& G = p->gl[l];
     const bss::SolveLevelArgs a{G.dchains, G.dtab[k], p->nrhs};
     hipError_t e;
@@ -638,14 +648,14 @@ int run_graph(bss_plan* p, hipGraphExec_t* ex, hipStream_t s, F enqueue) {
 
 // Mixed-mode FP32 correction solve after an FP64 separator re-factorisation: level 0
 // in FP32 (vector set k), separator levels in FP64, FP32 <-> FP64 at the hand-off.
-int solve_levels_sep64(bss_plan* p, int k, hipStream_t s) {
+int solve_levels_sep64(bss_plan* p, int k, hipStream_t s, bool l0_forward_done = false) {
   const int nb = p->n;
   const int nd = p->n * p->nrhs;
   const size_t L = p->gl.size();
   GLevel& G0 = p->gl[0];
   const bss::SolveLevelArgs a0{G0.dchains, G0.dtab[k], p->nrhs};
-  hipError_t e;
-  {
+  hipError_t e = hipSuccess;
+  if (!l0_forward_done) {
     ProfScope ps(p, BSS_K_FORWARD, 0, s);
     e = bss::launch_forward<float>(nb, a0, (int)G0.chains.size(), s);
   }
@@ -722,6 +732,42 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
   p->schedule = schedule;
   if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
   *out = p;
+  return BSS_OK;
+}
+
+// Refinement-loop hooks over this plan.  The first solve32 after bss_factorize_solve
+// (vector set 0, l0_done set) skips the level-0 forward sweep that already ran on the
+// side stream, and runs without a graph.
+bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
+  bss::MixedSolveHooks hooks;
+  hooks.ctx = p;
+  hooks.prepare = [](void* ctx, hipStream_t st) -> int { return ensure_pivots((bss_plan*)ctx, st); };
+  hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
+    bss_plan* q = (bss_plan*)ctx;
+    const bool skip = q->l0_done && k == 0;
+    q->l0_done = false;
+    if (q->sep64) {
+      if (skip) return solve_levels_sep64(q, k, st, true);
+      return run_graph(q, &q->g_solve64[k], st,
+                       [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
+    }
+    if (skip) return solve_levels<float>(q, k, st, 1);
+    return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
+  };
+  hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
+    return new ProfScope((bss_plan*)ctx, kind, 0, st);
+  };
+  hooks.prof_end = [](void* scope) { delete (ProfScope*)scope; };
+  return hooks;
+}
+
+int run_mixed_solve(bss_plan* p, hipStream_t s, bool first_pass_done) {
+  const bss::MixedSolveHooks hooks = mixed_hooks(p);
+  int iters = 0;
+  const int rc = bss::mixed_solve(p->mx, hooks, (double*)p->d, p->tol, p->max_iters, &iters, s,
+                                  first_pass_done);
+  p->last_iters = iters;
+  if (rc != 0) return fail(BSS_E_HIP, "mixed solve failed: %s", bss::mixed_last_error());
   return BSS_OK;
 }
 
@@ -924,30 +970,84 @@ int bss_solve(bss_plan_t p, void* stream) {
   if (!p->factored) return fail(BSS_E_STATE, "bss_factorize must precede bss_solve");
   hipStream_t s = (hipStream_t)stream;
   p->last_stream = s;
-  if (p->mode == BSS_MODE_MIXED) {
-    bss::MixedSolveHooks hooks;
-    hooks.ctx = p;
-    hooks.prepare = [](void* ctx, hipStream_t st) -> int { return ensure_pivots((bss_plan*)ctx, st); };
-    hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
-      bss_plan* q = (bss_plan*)ctx;
-      if (q->sep64)
-        return run_graph(q, &q->g_solve64[k], st,
-                         [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
-      return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
-    };
-    hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
-      return new ProfScope((bss_plan*)ctx, kind, 0, st);
-    };
-    hooks.prof_end = [](void* scope) { delete (ProfScope*)scope; };
-    int iters = 0;
-    int rc = bss::mixed_solve(p->mx, hooks, (double*)p->d, p->tol, p->max_iters, &iters, s);
-    p->last_iters = iters;
-    if (rc != 0) return fail(BSS_E_HIP, "mixed solve failed: %s", bss::mixed_last_error());
-    return BSS_OK;
-  }
+  p->l0_done = false;
+  if (p->mode == BSS_MODE_MIXED) return run_mixed_solve(p, s, false);
   if (p->dtype == BSS_F64)
     return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<double>(p, 0, c); });
   return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<float>(p, 0, c); });
+}
+
+int bss_factorize_solve(bss_plan_t p, void* stream) {
+  if (int e = check_plan(p)) return e;
+  if (!p->A) return fail(BSS_E_STATE, "bss_plan_bind must precede bss_factorize_solve");
+  if (p->use_graph || p->gl.size() < 2) {
+    if (int e = bss_factorize(p, stream)) return e;
+    return bss_solve(p, stream);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  p->last_stream = s;
+  p->factored = false;
+  p->l0_done = false;
+  if (!p->side_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_factored, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_solved, hipEventDisableTiming));
+  }
+  const bool mixed = p->mode == BSS_MODE_MIXED;
+  const bool f64 = !mixed && p->dtype == BSS_F64;
+  auto factor = [&](size_t lb, size_t le) {
+    if (mixed) return factor_levels<double, float>(p, s, lb, le);
+    if (f64) return factor_levels<double, double>(p, s, lb, le);
+    return factor_levels<float, float>(p, s, lb, le);
+  };
+  // levels 0 and 1 with their hand-offs on the caller's stream (ordered after the
+  // caller's writes of A, B and d); the side stream starts when they are done.  Level 1
+  // still fills the device, so the side stream's sweep only has idle compute units to
+  // use beside the later, latency-bound levels.
+  const size_t split = 2;
+  if (int e = factor(0, split)) return e;
+  hipStream_t s2 = p->side_stream;
+  HIP_TRY(hipEventRecord(p->ev_l0_factored, s));
+  HIP_TRY(hipStreamWaitEvent(s2, p->ev_l0_factored, 0));
+  // side stream: the first solve's level-0 forward sweep (mixed: after the x = 0
+  // residual pass that writes r = d in FP32) beside the separator levels' factorisation
+  const bss::MixedSolveHooks hooks = mixed_hooks(p);
+  if (mixed && bss::mixed_first_pass(p->mx, hooks, s2) != 0)
+    return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
+  GLevel& G0 = p->gl[0];
+  const bss::SolveLevelArgs a0{G0.dchains, G0.dtab[0], p->nrhs};
+  hipError_t e;
+  {
+    ProfScope ps(p, BSS_K_FORWARD, 0, s2);
+    e = f64 ? bss::launch_forward<double>(p->n, a0, (int)G0.chains.size(), s2)
+            : bss::launch_forward<float>(p->n, a0, (int)G0.chains.size(), s2);
+  }
+  // the join is recorded and awaited even after a failed launch, so that the caller's
+  // stream never runs ahead of side-stream work
+  const hipError_t ej = hipEventRecord(p->ev_l0_solved, s2);
+  const int rc = factor(split, ~size_t(0));
+  if (ej == hipSuccess) HIP_TRY(hipStreamWaitEvent(s, p->ev_l0_solved, 0));
+  if (e != hipSuccess) return fail(BSS_E_HIP, "level-0 forward sweep: %s", hipGetErrorString(e));
+  if (ej != hipSuccess) return fail(BSS_E_HIP, "side-stream join: %s", hipGetErrorString(ej));
+  if (rc != BSS_OK) return rc;
+  if (mixed) {
+    // FP32 separator info and pivot ratios to pinned memory, as bss_factorize does
+    HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(p->piv_event, s));
+    bss::mixed_after_factor(p->mx);
+    p->pivots_pending = true;
+    p->sep64 = false;
+  }
+  p->factored = true;
+  p->l0_done = true;
+  int rs;
+  if (mixed)
+    rs = run_mixed_solve(p, s, true);
+  else
+    rs = f64 ? solve_levels<double>(p, 0, s, 1) : solve_levels<float>(p, 0, s, 1);
+  p->l0_done = false;
+  return rs;
 }
 
 int bss_info(bss_plan_t p, int* info_host) {

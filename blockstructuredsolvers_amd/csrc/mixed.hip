@@ -593,15 +593,21 @@ void mixed_after_factor(MixedState& m) { m.mnorm_stale = true; }
 // below this tolerance the backward error after the first FP32 pass is not read
 constexpr double kFirstCheckTol = 1e-9;
 
-int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
-                int* iters, hipStream_t s) {
-  int cur = 0, xa = 0, it = 0;
-  // first pass: x = 0, r = d (as FP32 into v[0]), ||d||
-  hipError_t e = residual_prof(m, h, true, -1, -1, 0, 0, 0, s);
+int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s) {
+  // x = 0, r = d (as FP32 into v[0]), ||d||
+  const hipError_t e = residual_prof(m, h, true, -1, -1, 0, 0, 0, s);
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;
   }
+  return 0;
+}
+
+int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
+                int* iters, hipStream_t s, bool first_pass_done) {
+  int cur = 0, xa = 0, it = 0;
+  hipError_t e;
+  if (!first_pass_done && mixed_first_pass(m, h, s) != 0) return -1;
   if (h.prepare && h.prepare(h.ctx, s) != 0) {
     g_mixed_err = "solve preparation failed";
     return -1;

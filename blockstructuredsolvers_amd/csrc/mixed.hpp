@@ -83,7 +83,13 @@ void mixed_after_factor(MixedState& m);
 // max berr <= tol or max_iters solves.  On exit d holds x.  Synchronises the stream
 // once per iteration to test convergence.
 int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
-                int* iters, hipStream_t s);
+                int* iters, hipStream_t s, bool first_pass_done = false);
+
+// The refinement loop's first (x = 0) residual pass on its own: r = d as FP32 into the
+// level-0 vectors of set 0 and ||d||.  It reads only the caller's FP64 d, A and B, so it
+// may run beside the factorisation; mixed_solve(..., first_pass_done = true) then
+// starts at the first FP32 solve.
+int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s);
 
 const char* mixed_last_error();
 

@@ -402,6 +402,15 @@ def solve(data: BlockTriDiagData_batched) -> None:
         L.call("bss_solve", data._plan, _stream_handle(data.device))
 
 
+def factorize_solve(data: BlockTriDiagData_batched) -> None:
+    """``factorize!(data); solve!(data)`` (src/gpu_batched.jl:184-259) in one call, for
+    a d that is final before the factorisation: the first solve's level-0 forward
+    sweep overlaps the separator levels' factorisation on a second stream.  Same
+    results and state as ``factorize`` then ``solve``; asynchronous."""
+    with torch.cuda.device(data.device):
+        L.call("bss_factorize_solve", data._plan, _stream_handle(data.device))
+
+
 factorize_ = factorize
 solve_ = solve
 

@@ -173,6 +173,17 @@ int bss_factorize(bss_plan_t plan, void* stream);
  * iterative refinement and synchronises the stream once per iteration. */
 int bss_solve(bss_plan_t plan, void* stream);
 
+/* bss_factorize followed by bss_solve, with the same results and the same state
+ * afterwards (later bss_solve calls reuse the factors), for a caller whose d is
+ * final before the factorisation starts.  The first solve's level-0 forward sweep
+ * (and, in mixed mode, its x = 0 residual pass) runs on a plan-owned stream beside
+ * the factorisation of the separator levels, and is joined back into `stream`
+ * before the call returns its work.  Replaces the reference's back-to-back
+ * factorize!(data); solve!(data) (src/gpu_batched.jl:184-259, as called in
+ * test/test_rocm.jl:97-101); the reference has no combined call.  With BSS_OPT_GRAPH on,
+ * or a single level, it is exactly bss_factorize then bss_solve. */
+int bss_factorize_solve(bss_plan_t plan, void* stream);
+
 /* Build the block-tridiagonal normal equations of a linear state-space smoother on the
  * device: for the model x_{i+1} = F x_i + w (w ~ N(0, Q)), y_i = H x_i + v (v ~ N(0, R)),
  * A_i = Q^{-1} + F^T Q^{-1} F + H^T R^{-1} H (all N blocks) and B_i = -Q^{-1} F (N-1

@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 import torch
 
-from blockstructuredsolvers_amd import factorize, initialize_batched, solve, workloads
+from blockstructuredsolvers_amd import (factorize, factorize_solve, initialize_batched, solve,
+                                        workloads)
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -233,7 +234,7 @@ def test_factor_launch_variants_agree(mode, T):
                                          systems[k].d).max() <= 1e-12
 
 
-def run_gpu_group(systems, T=torch.float64, **kw):
+def run_gpu_group(systems, T=torch.float64, combined=False, **kw):
     from blockstructuredsolvers_amd import initialize_batched_group
     n, nrhs = systems[0].n, systems[0].nrhs
     data = initialize_batched_group([s.N for s in systems], n, T, "cuda", nrhs=nrhs, **kw)
@@ -242,8 +243,11 @@ def run_gpu_group(systems, T=torch.float64, **kw):
         data.A_tensors[s][:, :, :N].copy_(torch.from_numpy(sysm.A))
         data.B_tensors[s][:, :, :N - 1].copy_(torch.from_numpy(sysm.B))
         data.d_tensors[s][:, :, :N].copy_(torch.from_numpy(sysm.d))
-    factorize(data)
-    solve(data)
+    if combined:
+        factorize_solve(data)
+    else:
+        factorize(data)
+        solve(data)
     torch.cuda.synchronize()
     return [data.d_tensors[s][:, :, :sysm.N].double().cpu().numpy()
             for s, sysm in enumerate(systems)], data
@@ -353,6 +357,74 @@ def test_mixed_fp64_separator_refactor_group_graph():
         assert data.refactored == data.levels - 1
         for sysm, x in zip(systems, xs):
             assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
+
+
+
+@pytest.mark.parametrize("mode,T,nrhs,pivot_tol", [("plain", torch.float64, 1, None),
+                                                   ("plain", torch.float32, 2, None),
+                                                   ("mixed", torch.float64, 1, None),
+                                                   ("mixed", torch.float64, 3, 2.0)])
+def test_factorize_solve_matches_separate_calls(mode, T, nrhs, pivot_tol):
+    """factorize_solve (first solve's level-0 forward sweep on a side stream beside the
+    separator levels' factorisation) is bitwise factorize then solve, on new inputs
+    each time, with eager device work between calls, and leaves the same factors for
+    a later solve; pivot_tol = 2.0 forces the FP64 separator re-factorisation."""
+    N, n = 1500, 64
+    kw = dict(nrhs=nrhs, mode=mode, tuned=False, nsys=2)
+    if pivot_tol is not None:
+        kw["pivot_tol"] = pivot_tol
+    plans = {c: initialize_batched(N, n, T, "cuda", **kw) for c in (False, True)}
+    assert plans[True].levels > 2
+    junk = torch.randn(512, 512, device="cuda")
+    for rep in range(3):
+        systems = [workloads.spd_system(N, n, nrhs=nrhs, seed=900 + 10 * rep + s) for s in range(2)]
+        out = {}
+        for c, data in plans.items():
+            for s, sysm in enumerate(systems):
+                data.A_tensor[..., s].copy_(torch.from_numpy(sysm.A))
+                data.B_tensor[..., s].copy_(torch.from_numpy(sysm.B))
+                data.d_tensor[..., s].copy_(torch.from_numpy(sysm.d))
+            if c:
+                factorize_solve(data)
+            else:
+                factorize(data)
+                solve(data)
+            junk = torch.tanh(junk @ junk.T * 1e-3)
+            torch.cuda.synchronize()
+            out[c] = data.d_tensor.double().cpu().numpy()
+        assert np.array_equal(out[True], out[False]), rep
+        if mode == "mixed":
+            assert plans[True].refactored == plans[False].refactored
+            if pivot_tol:
+                assert plans[True].refactored == plans[True].levels - 1
+        assert plans[True].info().tolist() == plans[False].info().tolist() == [0, 0]
+        if mode == "mixed":
+            assert plans[True].iterations == plans[False].iterations
+        if T == torch.float64:
+            bound = 1e-13 if mode == "plain" else 1e-12
+            for s, sysm in enumerate(systems):
+                assert oracle.backward_error(sysm.A, sysm.B, out[True][..., s], sysm.d).max() <= bound
+    # a later solve on the factors left by factorize_solve
+    d2 = np.asfortranarray(np.random.default_rng(5).uniform(size=tuple(plans[True].d_tensor.shape)))
+    x2 = {}
+    for c, data in plans.items():
+        data.d_tensor.copy_(torch.from_numpy(d2))
+        solve(data)
+        torch.cuda.synchronize()
+        x2[c] = data.d_tensor.double().cpu().numpy()
+    assert np.array_equal(x2[True], x2[False])
+
+
+def test_factorize_solve_group_and_graph():
+    """factorize_solve on an uneven group (mixed mode), and with graph replay on (where
+    it is factorize then solve): bitwise the separate calls."""
+    systems = [workloads.spd_system(N, 32, nrhs=1, seed=950 + N) for N in (1, 90, 700)]
+    for graph in (False, True):
+        xs, _ = run_gpu_group(systems, mode="mixed", tuned=False, graph=graph)
+        ys, data = run_gpu_group(systems, mode="mixed", tuned=False, graph=graph, combined=True)
+        for x, y in zip(xs, ys):
+            assert np.array_equal(x, y)
+        assert data.info().tolist() == [0, 0, 0]
 
 
 def test_seq_solver_matches_oracle():

@@ -42,10 +42,12 @@ def test_argument_errors():
     rc = L.lib().bss_plan_create(10, 32, 17, 1, 1, 0, 0, ctypes.byref(p))
     assert rc == -5
     assert L.lib().bss_factorize(None, None) == -1
+    assert L.lib().bss_factorize_solve(None, None) == -1
     L.call("bss_plan_create", 10, 32, 1, 1, 1, 0, 0, ctypes.byref(p))
     try:
         assert L.lib().bss_factorize(p, None) == -4  # not bound
         assert L.lib().bss_solve(p, None) == -4      # not factored
+        assert L.lib().bss_factorize_solve(p, None) == -4  # not bound
         assert L.lib().bss_set_option(p, 99, 1.0) == -1
     finally:
         L.lib().bss_plan_destroy(p)
