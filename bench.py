@@ -135,6 +135,26 @@ def run_path(mode, systems, args, torch, bss):
     berr_dev = data.backward_error().max() if mode == "mixed" else None
     x0 = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
 
+    # the same steps through the combined entry point (first solve's level-0 sweep
+    # beside the factorisation's small levels), timed the same way; its answer must be
+    # bitwise the separate calls'
+    def step_combined():
+        data._d.copy_(d0)
+        bss.factorize_solve(data)
+
+    for _ in range(args.warmup):
+        step_combined()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_combined()
+    torch.cuda.synchronize()
+    wall_c = time.perf_counter() - t0
+    xc = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
+    combined = {"systems_per_s": nsys * args.steps / wall_c,
+                "ms_per_step": 1e3 * wall_c / args.steps,
+                "bitwise_equal_to_separate_calls": bool(np.array_equal(xc, x0))}
+
     # latency: repeated solves with resident factors (new RHS each time)
     lat = []
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -160,6 +180,7 @@ def run_path(mode, systems, args, torch, bss):
         "breakdown_steps": breakdown_steps,
         "levels": data.level_sizes(),
         "x0": x0,
+        "combined": combined,
     }
     data.close()
     return res
@@ -488,6 +509,7 @@ def main():
                              if plain and "mixed" in results else None),
         "refinement_iterations": results["mixed"]["iterations"] if "mixed" in results else None,
         "two_stream_pipelined_systems_per_s": {m: r.get("pipelined") for m, r in results.items()},
+        "factorize_solve_call": {m: r["combined"] for m, r in results.items()},
         "solve_ms_p50": {m: r["solve_ms_p50"] for m, r in results.items()},
         "solve_ms_p99": {m: r["solve_ms_p99"] for m, r in results.items()},
         "accuracy_vs_fp64_oracle": acc,
