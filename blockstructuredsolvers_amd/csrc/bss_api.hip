@@ -154,11 +154,11 @@ struct bss_plan {
   bool use_graph = false;
   hipStream_t cap_stream = nullptr;  // capture happens here, replay on the caller's stream
   hipGraphExec_t g_factor = nullptr, g_solve[2] = {nullptr, nullptr};
-  // bss_factorize_solve: the first solve's level-0 work runs on side_stream while the
-  // separator levels are factored; l0_done tells the next solve32 to skip it
+  // bss_factorize_solve: the first solve's forward sweeps of levels [0, fwd_done) run
+  // on side_stream while the later levels are factored; the next solve32 skips them
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_l0_factored = nullptr, ev_l0_solved = nullptr;
-  bool l0_done = false;
+  size_t fwd_done = 0;
   void drop_graphs() {
     for (hipGraphExec_t* g : {&g_factor, &g_solve[0], &g_solve[1], &g_solve64[0], &g_solve64[1]}) {
       if (*g) (void)hipGraphExecDestroy(*g);
@@ -736,22 +736,24 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
 }
 
 // Refinement-loop hooks over this plan.  The first solve32 after bss_factorize_solve
-// (vector set 0, l0_done set) skips the level-0 forward sweep that already ran on the
-// side stream, and runs without a graph.
+// (vector set 0, fwd_done set) skips the forward sweeps that already ran on the side
+// stream, and runs without a graph.  After an FP64 separator re-factorisation only the
+// FP32 level-0 sweep is reused (the FP32 separator sweeps wrote FP32 tables that the
+// FP64 separator solve does not read).
 bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
   bss::MixedSolveHooks hooks;
   hooks.ctx = p;
   hooks.prepare = [](void* ctx, hipStream_t st) -> int { return ensure_pivots((bss_plan*)ctx, st); };
   hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
     bss_plan* q = (bss_plan*)ctx;
-    const bool skip = q->l0_done && k == 0;
-    q->l0_done = false;
+    const size_t skip = k == 0 ? q->fwd_done : 0;
+    q->fwd_done = 0;
     if (q->sep64) {
       if (skip) return solve_levels_sep64(q, k, st, true);
       return run_graph(q, &q->g_solve64[k], st,
                        [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
     }
-    if (skip) return solve_levels<float>(q, k, st, 1);
+    if (skip) return solve_levels<float>(q, k, st, skip);
     return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
   };
   hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
@@ -970,7 +972,7 @@ int bss_solve(bss_plan_t p, void* stream) {
   if (!p->factored) return fail(BSS_E_STATE, "bss_factorize must precede bss_solve");
   hipStream_t s = (hipStream_t)stream;
   p->last_stream = s;
-  p->l0_done = false;
+  p->fwd_done = 0;
   if (p->mode == BSS_MODE_MIXED) return run_mixed_solve(p, s, false);
   if (p->dtype == BSS_F64)
     return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<double>(p, 0, c); });
@@ -987,7 +989,7 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   p->last_stream = s;
   p->factored = false;
-  p->l0_done = false;
+  p->fwd_done = 0;
   if (!p->side_stream) {
     HIP_TRY(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_factored, hipEventDisableTiming));
@@ -1002,32 +1004,33 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   };
   // levels 0 and 1 with their hand-offs on the caller's stream (ordered after the
   // caller's writes of A, B and d); the side stream starts when they are done.  Level 1
-  // still fills the device, so the side stream's sweep only has idle compute units to
-  // use beside the later, latency-bound levels.
+  // still fills the device, so the side stream's sweeps (levels 0 and 1) only have idle
+  // compute units to use beside the later, latency-bound levels.
   const size_t split = 2;
   if (int e = factor(0, split)) return e;
   hipStream_t s2 = p->side_stream;
   HIP_TRY(hipEventRecord(p->ev_l0_factored, s));
   HIP_TRY(hipStreamWaitEvent(s2, p->ev_l0_factored, 0));
-  // side stream: the first solve's level-0 forward sweep (mixed: after the x = 0
-  // residual pass that writes r = d in FP32) beside the separator levels' factorisation
+  // side stream: the first solve's forward sweeps of the factored levels (mixed: after
+  // the x = 0 residual pass that writes r = d in FP32) beside the later levels'
+  // factorisation
   const bss::MixedSolveHooks hooks = mixed_hooks(p);
   if (mixed && bss::mixed_first_pass(p->mx, hooks, s2) != 0)
     return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
-  GLevel& G0 = p->gl[0];
-  const bss::SolveLevelArgs a0{G0.dchains, G0.dtab[0], p->nrhs};
-  hipError_t e;
-  {
-    ProfScope ps(p, BSS_K_FORWARD, 0, s2);
-    e = f64 ? bss::launch_forward<double>(p->n, a0, (int)G0.chains.size(), s2)
-            : bss::launch_forward<float>(p->n, a0, (int)G0.chains.size(), s2);
+  hipError_t e = hipSuccess;
+  for (size_t l = 0; l < split && e == hipSuccess; ++l) {
+    GLevel& G = p->gl[l];
+    const bss::SolveLevelArgs a{G.dchains, G.dtab[0], p->nrhs};
+    ProfScope ps(p, BSS_K_FORWARD, (int)l, s2);
+    e = f64 ? bss::launch_forward<double>(p->n, a, (int)G.chains.size(), s2)
+            : bss::launch_forward<float>(p->n, a, (int)G.chains.size(), s2);
   }
   // the join is recorded and awaited even after a failed launch, so that the caller's
   // stream never runs ahead of side-stream work
   const hipError_t ej = hipEventRecord(p->ev_l0_solved, s2);
   const int rc = factor(split, ~size_t(0));
   if (ej == hipSuccess) HIP_TRY(hipStreamWaitEvent(s, p->ev_l0_solved, 0));
-  if (e != hipSuccess) return fail(BSS_E_HIP, "level-0 forward sweep: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(BSS_E_HIP, "side-stream forward sweep: %s", hipGetErrorString(e));
   if (ej != hipSuccess) return fail(BSS_E_HIP, "side-stream join: %s", hipGetErrorString(ej));
   if (rc != BSS_OK) return rc;
   if (mixed) {
@@ -1040,13 +1043,13 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
     p->sep64 = false;
   }
   p->factored = true;
-  p->l0_done = true;
+  p->fwd_done = split;
   int rs;
   if (mixed)
     rs = run_mixed_solve(p, s, true);
   else
-    rs = f64 ? solve_levels<double>(p, 0, s, 1) : solve_levels<float>(p, 0, s, 1);
-  p->l0_done = false;
+    rs = f64 ? solve_levels<double>(p, 0, s, split) : solve_levels<float>(p, 0, s, split);
+  p->fwd_done = 0;
   return rs;
 }
 
