@@ -158,6 +158,7 @@ struct bss_plan {
   // on side_stream while the later levels are factored; the next solve32 skips them
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_l0_factored = nullptr, ev_l0_solved = nullptr;
+  std::vector<hipEvent_t> ev_level;  // level l factored (levels beyond the first two)
   size_t fwd_done = 0;
   void drop_graphs() {
     for (hipGraphExec_t* g : {&g_factor, &g_solve[0], &g_solve[1], &g_solve64[0], &g_solve64[1]}) {
@@ -172,6 +173,7 @@ struct bss_plan {
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (ev_l0_factored) (void)hipEventDestroy(ev_l0_factored);
     if (ev_l0_solved) (void)hipEventDestroy(ev_l0_solved);
+    for (hipEvent_t e : ev_level) (void)hipEventDestroy(e);
     if (side_stream) (void)hipStreamDestroy(side_stream);
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
   }
@@ -995,6 +997,12 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
     HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_factored, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_solved, hipEventDisableTiming));
   }
+  const size_t L = p->gl.size();
+  while (p->ev_level.size() < L) {
+    hipEvent_t ev;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    p->ev_level.push_back(ev);
+  }
   const bool mixed = p->mode == BSS_MODE_MIXED;
   const bool f64 = !mixed && p->dtype == BSS_F64;
   auto factor = [&](size_t lb, size_t le) {
@@ -1004,8 +1012,9 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   };
   // levels 0 and 1 with their hand-offs on the caller's stream (ordered after the
   // caller's writes of A, B and d); the side stream starts when they are done.  Level 1
-  // still fills the device, so the side stream's sweeps (levels 0 and 1) only have idle
-  // compute units to use beside the later, latency-bound levels.
+  // still fills the device, so the side stream's sweeps (levels 0 and 1, then each
+  // later level once it is factored) only have idle compute units to use beside the
+  // later, latency-bound levels.
   const size_t split = 2;
   if (int e = factor(0, split)) return e;
   hipStream_t s2 = p->side_stream;
@@ -1025,30 +1034,49 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
     e = f64 ? bss::launch_forward<double>(p->n, a, (int)G.chains.size(), s2)
             : bss::launch_forward<float>(p->n, a, (int)G.chains.size(), s2);
   }
-  // the join is recorded and awaited even after a failed launch, so that the caller's
-  // stream never runs ahead of side-stream work
+  // each later level: factored on the caller's stream, then its forward sweep on the
+  // side stream while the next level is factored
+  int rc = BSS_OK;
+  for (size_t l = split; l < L && rc == BSS_OK && e == hipSuccess; ++l) {
+    rc = factor(l, l + 1);
+    if (rc != BSS_OK) break;
+    e = hipEventRecord(p->ev_level[l], s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s2, p->ev_level[l], 0);
+    if (e != hipSuccess) break;
+    GLevel& G = p->gl[l];
+    const bss::SolveLevelArgs a{G.dchains, G.dtab[0], p->nrhs};
+    ProfScope ps(p, BSS_K_FORWARD, (int)l, s2);
+    e = f64 ? bss::launch_forward<double>(p->n, a, (int)G.chains.size(), s2)
+            : bss::launch_forward<float>(p->n, a, (int)G.chains.size(), s2);
+  }
+  // mixed: FP32 separator info and pivot ratios to pinned memory, as bss_factorize
+  // does, before the join (the host's pivot check waits for the factorisation only)
+  hipError_t ep = hipSuccess;
+  if (mixed && rc == BSS_OK) {
+    ep = hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
+                        hipMemcpyDeviceToHost, s);
+    if (ep == hipSuccess) ep = hipEventRecord(p->piv_event, s);
+  }
+  // the join is recorded and awaited even after a failure, so that the caller's stream
+  // never runs ahead of side-stream work
   const hipError_t ej = hipEventRecord(p->ev_l0_solved, s2);
-  const int rc = factor(split, ~size_t(0));
   if (ej == hipSuccess) HIP_TRY(hipStreamWaitEvent(s, p->ev_l0_solved, 0));
   if (e != hipSuccess) return fail(BSS_E_HIP, "side-stream forward sweep: %s", hipGetErrorString(e));
   if (ej != hipSuccess) return fail(BSS_E_HIP, "side-stream join: %s", hipGetErrorString(ej));
   if (rc != BSS_OK) return rc;
+  if (ep != hipSuccess) return fail(BSS_E_HIP, "pivot copy: %s", hipGetErrorString(ep));
   if (mixed) {
-    // FP32 separator info and pivot ratios to pinned memory, as bss_factorize does
-    HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipEventRecord(p->piv_event, s));
     bss::mixed_after_factor(p->mx);
     p->pivots_pending = true;
     p->sep64 = false;
   }
   p->factored = true;
-  p->fwd_done = split;
+  p->fwd_done = L;
   int rs;
   if (mixed)
     rs = run_mixed_solve(p, s, true);
   else
-    rs = f64 ? solve_levels<double>(p, 0, s, split) : solve_levels<float>(p, 0, s, split);
+    rs = f64 ? solve_levels<double>(p, 0, s, L) : solve_levels<float>(p, 0, s, L);
   p->fwd_done = 0;
   return rs;
 }
