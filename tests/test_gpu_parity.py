@@ -360,16 +360,18 @@ def test_mixed_fp64_separator_refactor_group_graph():
 
 
 
-@pytest.mark.parametrize("mode,T,nrhs,pivot_tol", [("plain", torch.float64, 1, None),
-                                                   ("plain", torch.float32, 2, None),
-                                                   ("mixed", torch.float64, 1, None),
-                                                   ("mixed", torch.float64, 3, 2.0)])
-def test_factorize_solve_matches_separate_calls(mode, T, nrhs, pivot_tol):
+@pytest.mark.parametrize("mode,T,N,n,nrhs,pivot_tol",
+                         [("plain", torch.float64, 1500, 64, 1, None),
+                          ("plain", torch.float32, 1500, 64, 2, None),
+                          ("mixed", torch.float64, 1500, 64, 1, None),
+                          ("mixed", torch.float64, 1500, 64, 3, 2.0),
+                          ("mixed", torch.float64, 400, 128, 1, None),
+                          ("plain", torch.float64, 700, 16, 16, None)])
+def test_factorize_solve_matches_separate_calls(mode, T, N, n, nrhs, pivot_tol):
     """factorize_solve (first solve's level-0 forward sweep on a side stream beside the
     separator levels' factorisation) is bitwise factorize then solve, on new inputs
     each time, with eager device work between calls, and leaves the same factors for
     a later solve; pivot_tol = 2.0 forces the FP64 separator re-factorisation."""
-    N, n = 1500, 64
     kw = dict(nrhs=nrhs, mode=mode, tuned=False, nsys=2)
     if pivot_tol is not None:
         kw["pivot_tol"] = pivot_tol
