@@ -186,11 +186,12 @@ def run_path(mode, systems, args, torch, bss):
     return res
 
 
-def run_pipelined(mode, systems, args, torch, bss):
+def run_pipelined(mode, systems, args, torch, bss, combined=False):
     """Throughput with consecutive steps alternating between two plans on two HIP
     streams, each driven by its own host thread (the mixed loop's host waits are per
     stream), so one step's latency-bound small levels overlap the next step's top-level
-    kernels.  Same per-step work as run_path; no events.  Returns systems/s."""
+    kernels.  Same per-step work as run_path; no events.  With `combined`, each step is
+    one factorize_solve call.  Returns systems/s."""
     import threading
     N, n, nrhs, nsys = args.N, args.n, args.nrhs, len(systems)
     plans = []
@@ -211,8 +212,11 @@ def run_pipelined(mode, systems, args, torch, bss):
         with torch.cuda.stream(st):
             for _ in range(steps):
                 data._d.copy_(d0)
-                bss.factorize(data)
-                bss.solve(data)
+                if combined:
+                    bss.factorize_solve(data)
+                else:
+                    bss.factorize(data)
+                    bss.solve(data)
             st.synchronize()
 
     def run(steps_each):
@@ -439,6 +443,9 @@ def main():
             r["pipelined"] = run_pipelined(mode, systems, args, torch, bss)
             log(f"bench: {mode}: two plans on two streams, steps alternating: "
                 f"{r['pipelined']:.1f} systems/s")
+            r["pipelined_combined"] = run_pipelined(mode, systems, args, torch, bss, combined=True)
+            log(f"bench: {mode}: the same through factorize_solve: "
+                f"{r['pipelined_combined']:.1f} systems/s")
 
     acc = {}
     if not args.no_oracle:
@@ -509,6 +516,8 @@ def main():
                              if plain and "mixed" in results else None),
         "refinement_iterations": results["mixed"]["iterations"] if "mixed" in results else None,
         "two_stream_pipelined_systems_per_s": {m: r.get("pipelined") for m, r in results.items()},
+        "two_stream_pipelined_factorize_solve_systems_per_s":
+            {m: r.get("pipelined_combined") for m, r in results.items()},
         "factorize_solve_call": {m: r["combined"] for m, r in results.items()},
         "solve_ms_p50": {m: r["solve_ms_p50"] for m, r in results.items()},
         "solve_ms_p99": {m: r["solve_ms_p99"] for m, r in results.items()},
