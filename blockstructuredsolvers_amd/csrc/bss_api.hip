@@ -745,7 +745,15 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
 bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
   bss::MixedSolveHooks hooks;
   hooks.ctx = p;
-  hooks.prepare = [](void* ctx, hipStream_t st) -> int { return ensure_pivots((bss_plan*)ctx, st); };
+  // the first solve32 runs on the FP32 separators before the pivot check; an FP64
+  // re-factorisation asks for it to be redone
+  hooks.prepare = [](void* ctx, hipStream_t st) -> int {
+    bss_plan* q = (bss_plan*)ctx;
+    const bool was64 = q->sep64;
+    const int rc = ensure_pivots(q, st);
+    if (rc != BSS_OK) return rc;
+    return q->sep64 && !was64 ? 1 : 0;
+  };
   hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
     bss_plan* q = (bss_plan*)ctx;
     const size_t skip = k == 0 ? q->fwd_done : 0;

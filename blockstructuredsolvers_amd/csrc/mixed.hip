@@ -608,14 +608,27 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
   int cur = 0, xa = 0, it = 0;
   hipError_t e;
   if (!first_pass_done && mixed_first_pass(m, h, s) != 0) return -1;
-  if (h.prepare && h.prepare(h.ctx, s) != 0) {
-    g_mixed_err = "solve preparation failed";
-    return -1;
-  }
+  bool prepared = h.prepare == nullptr;
   while (true) {
     if (h.solve32(h.ctx, cur, s) != 0) {
       g_mixed_err = "FP32 solve failed";
       return -1;
+    }
+    if (!prepared) {
+      // the first solve is queued before the caller's check; a changed solve path
+      // redoes it (d is untouched until the end, so the x = 0 pass rebuilds r = d)
+      prepared = true;
+      const int pr = h.prepare(h.ctx, s);
+      if (pr == 1) {
+        if (mixed_first_pass(m, h, s) != 0) return -1;
+        if (h.solve32(h.ctx, cur, s) != 0) {
+          g_mixed_err = "FP32 solve failed";
+          return -1;
+        }
+      } else if (pr != 0) {
+        g_mixed_err = "solve preparation failed";
+        return -1;
+      }
     }
     const int want_mnorm = m.mnorm_stale ? 1 : 0;
     e = residual_prof(m, h, false, xa, cur, 1 - xa, 1 - cur, want_mnorm, s);

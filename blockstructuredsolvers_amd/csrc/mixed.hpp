@@ -67,8 +67,10 @@ struct MixedSolveHooks {
   void* ctx = nullptr;
   // FP32 recursive solve in place in the level-0 vectors of ping-pong set k
   int (*solve32)(void* ctx, int k, hipStream_t s) = nullptr;
-  // optional, called once after the first (x = 0) residual pass is enqueued and before
-  // the first solve32: the caller may wait on earlier work and choose the solve path
+  // optional, called once right after the first solve32 is enqueued, so that the first
+  // solve is queued behind earlier work while the caller waits on it: returns 0 when
+  // that solve stands, 1 when the caller has changed the solve path (the x = 0 pass and
+  // the first solve are then redone from d), anything else on failure
   int (*prepare)(void* ctx, hipStream_t s) = nullptr;
   // optional profiler brackets around the residual and backward-error launches
   void* (*prof_begin)(void* ctx, int kind, hipStream_t s) = nullptr;
