@@ -91,7 +91,7 @@ def run_path(mode, systems, args, torch, bss):
     # plain: factors to the workspace so A, B stay resident inputs across steps
     data = bss.initialize_batched(N, n, torch.float64, "cuda", nrhs=nrhs, nsys=nsys, mode=mode,
                                   schedule=args.schedule, chain_len=args.chain_len,
-                                  base_max=args.base_max, preserve_input=(mode == "plain"),
+                                  base_max=args.base_max,
                                   tol=args.tol if mode == "mixed" else None,
                                   chain_len_levels=args.chain_lens, factor_waves=args.factor_waves,
                                   graph=bool(args.graph))
@@ -135,24 +135,30 @@ def run_path(mode, systems, args, torch, bss):
     berr_dev = data.backward_error().max() if mode == "mixed" else None
     x0 = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
 
-    # the same steps through the combined entry point (first solve's level-0 sweep
-    # beside the factorisation's small levels), timed the same way; its answer must be
-    # bitwise the separate calls'
+    # the same steps through the combined entry point (first solve's forward sweeps
+    # beside the factorisation's small levels); its answer must be bitwise the separate
+    # calls'.  Both are timed here with the profiler off (no events at all), so the
+    # comparison is like for like.
     def step_combined():
         data._d.copy_(d0)
         bss.factorize_solve(data)
 
-    for _ in range(args.warmup):
-        step_combined()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step_combined()
-    torch.cuda.synchronize()
-    wall_c = time.perf_counter() - t0
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    wall_s = timed(step)
+    wall_c = timed(step_combined)
     xc = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
     combined = {"systems_per_s": nsys * args.steps / wall_c,
                 "ms_per_step": 1e3 * wall_c / args.steps,
+                "separate_calls_no_events_systems_per_s": nsys * args.steps / wall_s,
                 "bitwise_equal_to_separate_calls": bool(np.array_equal(xc, x0))}
 
     # latency: repeated solves with resident factors (new RHS each time)
@@ -199,7 +205,6 @@ def run_pipelined(mode, systems, args, torch, bss, combined=False):
         data = bss.initialize_batched(N, n, torch.float64, "cuda", nrhs=nrhs, nsys=nsys,
                                       mode=mode, schedule=args.schedule,
                                       chain_len=args.chain_len, base_max=args.base_max,
-                                      preserve_input=(mode == "plain"),
                                       tol=args.tol if mode == "mixed" else None,
                                       chain_len_levels=args.chain_lens,
                                       factor_waves=args.factor_waves)
@@ -403,7 +408,9 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="1: replay hipGraph-captured levels")
     ap.add_argument("--lat-solves", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: every core this process "
+                         "may run on)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--modes", default="plain,mixed")
@@ -431,9 +438,10 @@ def main():
         log(f"bench: {mode} path")
         results[mode] = run_path(mode, systems, args, torch, bss)
         r = results[mode]
+        lat = (f"solve p50 {r['solve_ms_p50']:.3f} ms p99 {r['solve_ms_p99']:.3f} ms"
+               if r["solve_ms_p50"] is not None else "no latency solves")
         log(f"bench: {mode}: {r['systems_per_s']:.1f} systems/s, {r['ms_per_step']:.3f} ms/step, "
-            f"solve p50 {r['solve_ms_p50']:.3f} ms p99 {r['solve_ms_p99']:.3f} ms, "
-            f"iters {r['iterations']}")
+            f"{lat}, iters {r['iterations']}")
         for (k, lvl), (c, t) in sorted(r["kernels_all"].items(), key=lambda kv: -kv[1][1]):
             log(f"    {k:14s} L{lvl}: {c:4d} launches, {t / c:.4f} ms avg, {t:.3f} ms total")
         for (k, lvl), (c, t) in r["kernels"].items():
@@ -462,13 +470,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline, {threads} threads, {args.cpu_seconds}s")
+        # every core this process may run on, within the CPU share the host allots a
+        # job (OMP_NUM_THREADS, when set)
+        host_cores = len(os.sched_getaffinity(0))
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host_cores
+        threads = args.cpu_threads or min(host_cores, share)
+        log(f"bench: CPU baseline, {threads} threads ({host_cores} cores in this process's "
+            f"affinity mask, CPU share {share}), {args.cpu_seconds}s")
         v, count, el = cpu_baseline(systems[0], args.cpu_seconds, threads)
-        cpu = {"value": v, "unit": "systems/s", "cores": threads, "kind": "port",
+        cpu = {"value": v, "unit": "systems/s", "cores": threads, "host_cores": host_cores,
+               "cpu_share": share, "kind": "port",
                "sample": f"{count} FP64 factorize+solve of one N={args.N}, n={args.n}, "
-                         f"nrhs={args.nrhs} system in {el:.1f} s, one system per thread "
-                         "(C restatement of cpu.jl, oracle/bss_oracle.c)"}
+                         f"nrhs={args.nrhs} system in {el:.1f} s, one system per thread on "
+                         f"{threads} of the {host_cores} cores this process may use (CPU "
+                         f"share {share}) (C restatement of "
+                         "cpu.jl, oracle/bss_oracle.c, unblocked loops, no BLAS)"}
 
     if args.calib_copy:
         # a plain copy of known size: HBM counters of the kernels above are scaled by

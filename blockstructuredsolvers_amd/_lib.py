@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libbss.so")
 BSS_F32, BSS_F64 = 0, 1
 BSS_MODE_PLAIN, BSS_MODE_MIXED = 0, 1
 BSS_SCHED_AUTO, BSS_SCHED_REFERENCE = 0, 1
+BSS_OK, BSS_NOT_CONVERGED = 0, 1
 
 BSS_Q_N_USE = 0
 BSS_Q_LEVELS = 1
@@ -24,6 +25,7 @@ BSS_Q_LEVEL_CHAINS = 48
 BSS_Q_LEVEL_MAXLEN = 80
 BSS_Q_LAST_ITERS = 112
 BSS_Q_REFACTORED = 113
+BSS_Q_FALLBACK = 114
 
 BSS_OPT_TOL = 0
 BSS_OPT_MAX_ITERS = 1
@@ -35,6 +37,7 @@ BSS_OPT_FACTOR_WAVES = 6
 BSS_OPT_GRAPH = 7
 BSS_OPT_PROFILE_KINDS = 8
 BSS_OPT_PROFILE_MAX_LEVEL = 9
+BSS_OPT_FALLBACK = 10
 BSS_OPT_CHAIN_LEN_LEVEL = 32
 
 KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,

@@ -69,7 +69,7 @@ def _time_config(cfg, systems, dtype, mode, reps, bss):
     s0 = systems[0]
     N, n, nrhs, S = s0.N, s0.n, s0.nrhs, len(systems)
     data = initialize_batched(N, n, dtype, "cuda", nrhs=nrhs, nsys=S, mode=mode,
-                              preserve_input=(mode == "plain"), **cfg)
+                              **cfg)
     for k, sk in enumerate(systems):
         view = (lambda t: t) if S == 1 else (lambda t, k=k: t[..., k])
         view(data.A_tensor).copy_(torch.from_numpy(sk.A).to(dtype))

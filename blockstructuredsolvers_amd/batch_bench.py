@@ -53,20 +53,23 @@ def fill_group(data, Ns, n, seed: int):
         data.d_tensors[s].copy_(d.to(dt).permute(2, 1, 0))
 
 
-def run(mode: str, groups, steps: int, warmup: int, seed: int, check: int):
-    """Time `steps` batch steps; returns (plans, ms per step, sampled right-hand sides)."""
+def run(mode: str, groups, steps: int, warmup: int, seed: int, check: int, picks=None):
+    """Time `steps` batch steps; returns (plans, ms per step, sampled right-hand sides):
+    `check` systems drawn at random, or the (group, system) pairs of `picks`."""
     plans, streams, dsave, samples = [], [], [], []
     rng = np.random.default_rng(seed)
     for gi, (n, Ns) in enumerate(groups):
         data = initialize_batched_group(Ns, n, torch.float64, "cuda", mode=mode,
-                                        preserve_input=(mode == "plain"), tuned=False)
+                                        tuned=False)
         fill_group(data, Ns, n, seed + gi)
         plans.append(data)
         streams.append(torch.cuda.Stream())
         dsave.append(data._d.clone())
         # a few systems of each group for the backward-error check: their inputs now
-        for s in rng.permutation(len(Ns))[:max(1, check // len(groups))]:
-            samples.append((gi, int(s), data.d_tensors[int(s)].double().cpu().numpy()))
+        chosen = ([s for g, s in picks if g == gi] if picks is not None else
+                  [int(s) for s in rng.permutation(len(Ns))[:max(1, check // len(groups))]])
+        for s in chosen:
+            samples.append((gi, s, data.d_tensors[s].double().cpu().numpy()))
     torch.cuda.synchronize()
 
     def step():

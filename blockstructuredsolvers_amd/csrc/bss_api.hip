@@ -115,7 +115,17 @@ struct bss_plan {
   int chain_len_default = 3;
   int base_max = 8;
   int factor_waves = 4;
-  bool preserve = false;  // plain mode: factors to workspace, A/B untouched
+  bool preserve = true;  // plain mode: factors to workspace, A/B untouched
+  // bss_info: per-system LAPACK-order info of the last factorisation, computed once
+  std::vector<int> info_exact;
+  bool info_valid = false;
+  // mixed mode: plain FP64 plan over the same systems and buffers (input preserved),
+  // used when refinement stops short (BSS_OPT_FALLBACK)
+  bool fallback_on = true;
+  bss_plan* fb = nullptr;
+  bool fb_factored = false;  // fb holds the factors of the current A, B
+  bool fb_active = false;    // solves of this factorisation go through fb
+  bool fp32_failed = false;  // an FP32 level-0 pivot failed in the last factorisation
   // mixed-precision state
   bss::MixedState mx;
   size_t mixed_off = kNone;
@@ -145,8 +155,9 @@ struct bss_plan {
   bss::ConvRec<double, float>* dnarrow = nullptr;
   int nconv = 0;
   bool pivots_pending = false;  // factorized, pivots not yet inspected
-  unsigned* host_piv = nullptr;  // pinned: [0, nsys) FP32 separator info words,
-                                 // [nsys, 2 nsys) pivot ratios, copied after each factorize
+  unsigned* host_piv = nullptr;  // pinned: [0, nsys) FP32 level-0 info words, [nsys,
+                                 // 2 nsys) FP32 separator info words, [2 nsys, 3 nsys)
+                                 // pivot ratios, copied after each factorize
   hipEvent_t piv_event = nullptr;
   bool sep64 = false;           // separator levels currently factored in FP64
   hipGraphExec_t g_solve64[2] = {nullptr, nullptr};
@@ -490,7 +501,7 @@ int build(bss_plan* p) {
                     (long long)V.shape.N * nd});
     }
     p->nconv = (int)wr.size();
-    HIP_TRY(hipHostMalloc((void**)&p->host_piv, sizeof(unsigned) * 2 * p->nsys));
+    HIP_TRY(hipHostMalloc((void**)&p->host_piv, sizeof(unsigned) * 3 * p->nsys));
     HIP_TRY(hipEventCreateWithFlags(&p->piv_event, hipEventDisableTiming));
     if (!wr.empty()) {
       HIP_TRY(hipMemcpy(p->dwiden, wr.data(), wr.size() * sizeof(wr[0]), hipMemcpyHostToDevice));
@@ -575,13 +586,16 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
   p->pivots_pending = false;
   p->sep64 = false;
   p->refactored = 0;
-  if (p->gl.size() < 2) return BSS_OK;
+  p->fp32_failed = false;
   HIP_TRY(bss::wait_event_spin(p->piv_event));
+  for (int q = 0; q < p->nsys; ++q) p->fp32_failed |= (int)p->host_piv[q] != kInfoNone;
+  // an FP32 level-0 failure cannot be refined away: the solve goes to the fallback
+  if (p->fp32_failed || p->gl.size() < 2) return BSS_OK;
   bool need = false;
   for (int q = 0; q < p->nsys; ++q) {
     float r;
-    memcpy(&r, &p->host_piv[p->nsys + q], sizeof(r));
-    if (!(r >= (float)p->pivot_tol) || (int)p->host_piv[q] != kInfoNone) need = true;
+    memcpy(&r, &p->host_piv[2 * p->nsys + q], sizeof(r));
+    if (!(r >= (float)p->pivot_tol) || (int)p->host_piv[p->nsys + q] != kInfoNone) need = true;
   }
   if (!need) return BSS_OK;
   if (int e = refactor_separators64(p, s)) return e;
@@ -650,14 +664,14 @@ int run_graph(bss_plan* p, hipGraphExec_t* ex, hipStream_t s, F enqueue) {
 
 // Mixed-mode FP32 correction solve after an FP64 separator re-factorisation: level 0
 // in FP32 (vector set k), separator levels in FP64, FP32 <-> FP64 at the hand-off.
-int solve_levels_sep64(bss_plan* p, int k, hipStream_t s, bool l0_forward_done = false) {
+int solve_levels_sep64(bss_plan* p, int k, hipStream_t s) {
   const int nb = p->n;
   const int nd = p->n * p->nrhs;
   const size_t L = p->gl.size();
   GLevel& G0 = p->gl[0];
   const bss::SolveLevelArgs a0{G0.dchains, G0.dtab[k], p->nrhs};
-  hipError_t e = hipSuccess;
-  if (!l0_forward_done) {
+  hipError_t e;
+  {
     ProfScope ps(p, BSS_K_FORWARD, 0, s);
     e = bss::launch_forward<float>(nb, a0, (int)G0.chains.size(), s);
   }
@@ -739,30 +753,30 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
 
 // Refinement-loop hooks over this plan.  The first solve32 after bss_factorize_solve
 // (vector set 0, fwd_done set) skips the forward sweeps that already ran on the side
-// stream, and runs without a graph.  After an FP64 separator re-factorisation only the
-// FP32 level-0 sweep is reused (the FP32 separator sweeps wrote FP32 tables that the
-// FP64 separator solve does not read).
+// stream, and runs without a graph.  That first solve32 always runs on the FP32
+// separators: the pivot check (prepare) follows it, and an FP64 separator
+// re-factorisation there makes the loop redo it from d on the FP64 separators.
 bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
   bss::MixedSolveHooks hooks;
   hooks.ctx = p;
   // the first solve32 runs on the FP32 separators before the pivot check; an FP64
-  // re-factorisation asks for it to be redone
+  // re-factorisation asks for it to be redone, a failed FP32 level-0 pivot for the
+  // refinement to stop
   hooks.prepare = [](void* ctx, hipStream_t st) -> int {
     bss_plan* q = (bss_plan*)ctx;
     const bool was64 = q->sep64;
     const int rc = ensure_pivots(q, st);
     if (rc != BSS_OK) return rc;
+    if (q->fp32_failed) return 2;
     return q->sep64 && !was64 ? 1 : 0;
   };
   hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
     bss_plan* q = (bss_plan*)ctx;
     const size_t skip = k == 0 ? q->fwd_done : 0;
     q->fwd_done = 0;
-    if (q->sep64) {
-      if (skip) return solve_levels_sep64(q, k, st, true);
+    if (q->sep64)
       return run_graph(q, &q->g_solve64[k], st,
                        [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
-    }
     if (skip) return solve_levels<float>(q, k, st, skip);
     return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
   };
@@ -773,14 +787,106 @@ bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
   return hooks;
 }
 
+size_t packed_vector_elems(const bss_plan* p) {
+  size_t nd = 0;
+  for (const SysPlan& S : p->sp) nd += (size_t)S.N_use * p->n * p->nrhs;
+  return nd;
+}
+
+// Mixed mode, refinement stopped short: solve with a plain FP64 factorisation of the
+// same systems (created and bound on first use, factored once per factorisation).  It
+// solves in iterate set 0 (d copied there first, so the caller's d is the right-hand
+// side of the backward-error pass), the answer's backward error is measured in FP64 by
+// the refinement's own residual pass, and the answer goes to d.
+int fallback_solve(bss_plan* p, hipStream_t s) {
+  bss::MixedState& m = p->mx;
+  double* x0 = bss::mixed_iterate(m, 0);
+  if (!p->fb) {
+    bss_plan* q = nullptr;
+    if (int e = create(p->nsys, p->Ns.data(), p->n, p->nrhs, BSS_F64, BSS_MODE_PLAIN,
+                       p->schedule, &q))
+      return e;
+    q->chain_len = p->chain_len;
+    q->chain_len_default = p->chain_len_default;
+    q->base_max = p->base_max;
+    q->factor_waves = p->factor_waves;
+    q->preserve = true;
+    if (int e = bss_plan_bind(q, p->A, p->B, x0)) {
+      bss_plan_destroy(q);
+      return e;
+    }
+    p->fb = q;
+    p->fb_factored = false;
+  }
+  const size_t nd = packed_vector_elems(p);
+  HIP_TRY(hipMemcpyAsync(x0, p->d, nd * 8, hipMemcpyDeviceToDevice, s));
+  if (!p->fb_factored) {
+    if (int e = bss_factorize(p->fb, s)) return e;
+    p->fb_factored = true;
+  }
+  if (int e = bss_solve(p->fb, s)) return e;
+  double worst = 0.0;
+  if (bss::mixed_measure(m, mixed_hooks(p), s, &worst) != 0)
+    return fail(BSS_E_HIP, "FP64 fallback backward error: %s", bss::mixed_last_error());
+  HIP_TRY(hipMemcpyAsync(p->d, bss::mixed_iterate(m, 1), nd * 8, hipMemcpyDeviceToDevice, s));
+  return worst <= p->tol ? BSS_OK : BSS_NOT_CONVERGED;
+}
+
 int run_mixed_solve(bss_plan* p, hipStream_t s, bool first_pass_done) {
+  if (p->fb_active) {
+    p->last_iters = 0;
+    p->fwd_done = 0;
+    return fallback_solve(p, s);
+  }
   const bss::MixedSolveHooks hooks = mixed_hooks(p);
   int iters = 0;
-  const int rc = bss::mixed_solve(p->mx, hooks, (double*)p->d, p->tol, p->max_iters, &iters, s,
-                                  first_pass_done);
+  bool converged = false;
+  const int rc = bss::mixed_solve(p->mx, hooks, (double*)p->d, p->tol, p->max_iters, &iters,
+                                  &converged, !p->fallback_on, s, first_pass_done);
   p->last_iters = iters;
   if (rc != 0) return fail(BSS_E_HIP, "mixed solve failed: %s", bss::mixed_last_error());
-  return BSS_OK;
+  if (converged) return BSS_OK;
+  if (!p->fallback_on) return BSS_NOT_CONVERGED;
+  p->fb_active = true;
+  return fallback_solve(p, s);
+}
+
+// LAPACK-order info of system s: its blocks factored again as one chain from block 0
+// (the sequential order of src/backbone_cpu.jl:5-15) from the preserved input, in FP64
+// for a mixed plan, into a temporary plan's workspace; the chain reports its first
+// failed block.  Only systems the recursive factorisation flagged come here.
+int sequential_info(bss_plan* p, int s, hipStream_t st, int* out) {
+  const int NU = p->sp[s].N_use;
+  const size_t nn = (size_t)p->n * p->n, nd = (size_t)p->n * p->nrhs;
+  size_t oa = 0, ob = 0, od = 0;
+  for (int q = 0; q < s; ++q) {
+    oa += (size_t)p->sp[q].N_use * nn;
+    ob += (size_t)std::max(p->sp[q].N_use - 1, 0) * nn;
+    od += (size_t)p->sp[q].N_use * nd;
+  }
+  const size_t is = isize(p);
+  const int dt = p->mode == BSS_MODE_MIXED ? BSS_F64 : p->dtype;
+  bss_plan* q = nullptr;
+  if (int e = create(1, &NU, p->n, 1, dt, BSS_MODE_PLAIN, BSS_SCHED_AUTO, &q)) return e;
+  q->base_max = NU;  // one level: a single chain over all blocks
+  q->preserve = true;
+  // the chain's factorisation reads A and B only; d is bound because a plan needs one
+  int rc = bss_plan_bind(q, (char*)p->A + oa * is,
+                         NU > 1 ? (void*)((char*)p->B + ob * is) : nullptr,
+                         (char*)p->d + od * is);
+  if (rc == BSS_OK) rc = bss_factorize(q, st);
+  int v = kInfoNone;
+  if (rc == BSS_OK) {
+    const hipError_t e1 = hipStreamSynchronize(st);
+    const hipError_t e2 = e1 == hipSuccess ? hipMemcpy(&v, q->dinfo, sizeof(int),
+                                                       hipMemcpyDeviceToHost)
+                                           : e1;
+    if (e2 != hipSuccess)
+      rc = fail(BSS_E_HIP, "sequential re-check: %s", hipGetErrorString(e2));
+  }
+  bss_plan_destroy(q);
+  *out = v == kInfoNone ? 0 : v;
+  return rc;
 }
 
 }  // namespace
@@ -870,6 +976,9 @@ int bss_set_option(bss_plan_t p, int option, double value) {
       if (value < 0) return fail(BSS_E_ARG, "profile max level must be >= 0");
       p->prof_max_level = value > (double)(1 << 30) ? 1 << 30 : (int)value;
       return BSS_OK;
+    case BSS_OPT_FALLBACK:
+      p->fallback_on = value != 0.0;
+      return BSS_OK;
     default:
       if (option >= BSS_OPT_CHAIN_LEN_LEVEL && option < BSS_OPT_CHAIN_LEN_LEVEL + 16) {
         if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "chain length must be >= 1");
@@ -908,6 +1017,8 @@ int bss_plan_get(bss_plan_t p, int query, int64_t* value) {
     *value = p->last_iters;
   } else if (query == BSS_Q_REFACTORED) {
     *value = p->refactored;
+  } else if (query == BSS_Q_FALLBACK) {
+    *value = p->fb_active ? 1 : 0;
   } else {
     return fail(BSS_E_ARG, "unknown query %d", query);
   }
@@ -939,6 +1050,12 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
   p->A = A;
   p->B = B;
   p->d = d;
+  if (p->fb) {  // the FP64 fallback plan names the previous buffers
+    bss_plan_destroy(p->fb);
+    p->fb = nullptr;
+  }
+  p->fb_active = p->fb_factored = false;
+  p->info_valid = false;
   if (p->mode == BSS_MODE_MIXED && p->mx.bind((const double*)A, (const double*)B, (const double*)d) != 0)
     return fail(BSS_E_HIP, "mixed bind: %s", bss::mixed_last_error());
   // levels 0 and 1 name the caller's buffers (level 1 reads level 0's vectors)
@@ -954,12 +1071,14 @@ int bss_factorize(bss_plan_t p, void* stream) {
   if (!p->A) return fail(BSS_E_STATE, "bss_plan_bind must precede bss_factorize");
   hipStream_t s = (hipStream_t)stream;
   p->last_stream = s;
+  p->info_valid = false;
+  p->fb_active = p->fb_factored = false;
   int rc;
   if (p->mode == BSS_MODE_MIXED) {
     rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, float>(p, c); });
-    if (rc == BSS_OK && p->gl.size() > 1) {
-      // FP32 separator info and pivot ratios to pinned memory; read at the first solve
-      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
+    if (rc == BSS_OK) {
+      // FP32 info words and pivot ratios to pinned memory; read at the first solve
+      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo, sizeof(int) * 3 * p->nsys,
                              hipMemcpyDeviceToHost, s));
       HIP_TRY(hipEventRecord(p->piv_event, s));
     }
@@ -1000,6 +1119,8 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   p->last_stream = s;
   p->factored = false;
   p->fwd_done = 0;
+  p->info_valid = false;
+  p->fb_active = p->fb_factored = false;
   if (!p->side_stream) {
     HIP_TRY(hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&p->ev_l0_factored, hipEventDisableTiming));
@@ -1057,11 +1178,11 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
     e = f64 ? bss::launch_forward<double>(p->n, a, (int)G.chains.size(), s2)
             : bss::launch_forward<float>(p->n, a, (int)G.chains.size(), s2);
   }
-  // mixed: FP32 separator info and pivot ratios to pinned memory, as bss_factorize
-  // does, before the join (the host's pivot check waits for the factorisation only)
+  // mixed: FP32 info words and pivot ratios to pinned memory, as bss_factorize does,
+  // before the join (the host's pivot check waits for the factorisation only)
   hipError_t ep = hipSuccess;
   if (mixed && rc == BSS_OK) {
-    ep = hipMemcpyAsync(p->host_piv, p->dinfo + p->nsys, sizeof(int) * 2 * p->nsys,
+    ep = hipMemcpyAsync(p->host_piv, p->dinfo, sizeof(int) * 3 * p->nsys,
                         hipMemcpyDeviceToHost, s);
     if (ep == hipSuccess) ep = hipEventRecord(p->piv_event, s);
   }
@@ -1093,16 +1214,28 @@ int bss_info(bss_plan_t p, int* info_host) {
   if (int e = check_plan(p)) return e;
   if (!info_host) return fail(BSS_E_ARG, "info_host is null");
   if (!p->built || !p->factored) return fail(BSS_E_STATE, "no factorization yet");
-  if (int e = ensure_pivots(p, p->last_stream)) return e;
-  HIP_TRY(hipStreamSynchronize(p->last_stream));
-  std::vector<int> info(2 * p->nsys);
-  HIP_TRY(hipMemcpy(info.data(), p->dinfo, sizeof(int) * 2 * p->nsys, hipMemcpyDeviceToHost));
-  for (int s = 0; s < p->nsys; ++s) {
-    // FP32 separator failures count unless the separator levels were re-factored in FP64
-    int v = info[s];
-    if (p->mode == BSS_MODE_MIXED && !p->sep64) v = std::min(v, info[p->nsys + s]);
-    info_host[s] = v == kInfoNone ? 0 : v;
+  if (!p->info_valid) {
+    if (int e = ensure_pivots(p, p->last_stream)) return e;
+    HIP_TRY(hipStreamSynchronize(p->last_stream));
+    std::vector<int> info(2 * p->nsys);
+    HIP_TRY(hipMemcpy(info.data(), p->dinfo, sizeof(int) * 2 * p->nsys, hipMemcpyDeviceToHost));
+    p->info_exact.assign(p->nsys, 0);
+    for (int s = 0; s < p->nsys; ++s) {
+      // FP32 separator failures count unless the separator levels were re-factored in
+      // FP64 (whose failures land in the first word, with level 0's)
+      int v = info[s];
+      if (p->mode == BSS_MODE_MIXED && !p->sep64) v = std::min(v, info[p->nsys + s]);
+      if (v == kInfoNone) continue;
+      // the recursive order meets a failure at some block of a non-SPD system; the
+      // LAPACK-order index needs the sequential re-check (preserved input only)
+      if (inplace0(p))
+        p->info_exact[s] = v;
+      else if (int e = sequential_info(p, s, p->last_stream, &p->info_exact[s]))
+        return e;
+    }
+    p->info_valid = true;
   }
+  for (int s = 0; s < p->nsys; ++s) info_host[s] = p->info_exact[s];
   return BSS_OK;
 }
 
@@ -1136,6 +1269,7 @@ int bss_plan_set_workspace(bss_plan_t p, void* ws, size_t bytes) {
 
 int bss_plan_destroy(bss_plan_t p) {
   if (!p) return BSS_OK;
+  if (p->fb) bss_plan_destroy(p->fb);
   if (p->ws) {
     (void)hipStreamSynchronize(p->last_stream);
     if (p->ws_owned) (void)hipFree(p->ws);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <string>
 
 #include "common.hpp"
@@ -603,12 +604,51 @@ int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s) {
   return 0;
 }
 
+namespace {
+// copy the packed iterate set k into the caller's d
+int write_iterate(MixedState& m, double* d, int k, hipStream_t s) {
+  size_t nd_all = 0;
+  for (int sy = 0; sy < m.nsys; ++sy) nd_all += (size_t)m.Ns[sy] * m.n * m.nrhs;
+  const hipError_t e = hipMemcpyAsync(d, m.sys_host[0].x[k], nd_all * 8,
+                                      hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) {
+    g_mixed_err = hipGetErrorString(e);
+    return -1;
+  }
+  return 0;
+}
+
+// berr to the host (one small copy, polled), into last_berr; *worst = max (NaN wins)
+int read_berr(MixedState& m, hipStream_t s, double* worst) {
+  hipError_t e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * m.nsys * m.nrhs,
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipEventRecord(m.berr_ready, s);
+  if (e == hipSuccess) e = wait_event_spin(m.berr_ready);
+  if (e != hipSuccess) {
+    g_mixed_err = hipGetErrorString(e);
+    return -1;
+  }
+  double w = 0.0;
+  for (int k = 0; k < m.nsys * m.nrhs; ++k) {
+    const double b = m.host_berr[k];
+    m.last_berr[k] = b;
+    w = (b != b || w != w) ? NAN : std::max(w, b);
+  }
+  *worst = w;
+  return 0;
+}
+}  // namespace
+
 int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
-                int* iters, hipStream_t s, bool first_pass_done) {
+                int* iters, bool* converged, bool write_unconverged, hipStream_t s,
+                bool first_pass_done) {
   int cur = 0, xa = 0, it = 0;
   hipError_t e;
+  *converged = false;
+  *iters = 0;
   if (!first_pass_done && mixed_first_pass(m, h, s) != 0) return -1;
   bool prepared = h.prepare == nullptr;
+  double prev = HUGE_VAL;  // backward error at the previous check
   while (true) {
     if (h.solve32(h.ctx, cur, s) != 0) {
       g_mixed_err = "FP32 solve failed";
@@ -625,6 +665,10 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
           g_mixed_err = "FP32 solve failed";
           return -1;
         }
+      } else if (pr == 2) {
+        // an FP32 pivot failed: every iterate would be NaN
+        for (double& b : m.last_berr) b = NAN;
+        return 0;
       } else if (pr != 0) {
         g_mixed_err = "solve preparation failed";
         return -1;
@@ -633,21 +677,6 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
     const int want_mnorm = m.mnorm_stale ? 1 : 0;
     e = residual_prof(m, h, false, xa, cur, 1 - xa, 1 - cur, want_mnorm, s);
     m.mnorm_stale = false;
-    // One FP32 solve of d cannot reach a tolerance far below FP32's precision, so the
-    // first pass is not checked: the host queues the next correction while the device
-    // is still busy instead of waiting for the backward error (an exactly solvable
-    // system then takes one more, harmless, correction).
-    if (e == hipSuccess && it == 0 && tol < kFirstCheckTol && max_iters > 1) {
-      xa = 1 - xa;
-      cur = 1 - cur;
-      ++it;
-      continue;
-    }
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(m.host_berr, m.berr, sizeof(double) * m.nsys * m.nrhs,
-                         hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipEventRecord(m.berr_ready, s);
-    if (e == hipSuccess) e = wait_event_spin(m.berr_ready);
     if (e != hipSuccess) {
       g_mixed_err = hipGetErrorString(e);
       return -1;
@@ -655,26 +684,41 @@ int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, 
     xa = 1 - xa;
     cur = 1 - cur;
     ++it;
-    double worst = 0.0;
-    bool nan = false;
-    for (int k = 0; k < m.nsys * m.nrhs; ++k) {
-      const double b = m.host_berr[k];
-      m.last_berr[k] = b;
-      if (b != b) nan = true;
-      worst = std::max(worst, b);
+    // One FP32 solve of d cannot reach a tolerance far below FP32's precision, so the
+    // first pass is not checked: the host queues the next correction while the device
+    // is still busy instead of waiting for the backward error (an exactly solvable
+    // system then takes one more, harmless, correction).
+    if (it == 1 && tol < kFirstCheckTol && max_iters > 1) continue;
+    double worst;
+    if (read_berr(m, s, &worst) != 0) return -1;
+    if (worst <= tol) {
+      *converged = true;
+      break;
     }
-    if ((!nan && worst <= tol) || it >= max_iters) break;
+    // stop short: NaN (never compares), the iteration cap, or stagnation -- refinement
+    // with a contraction factor above 1/2 is not going to reach the target soon, and
+    // above 1 it diverges
+    if (worst != worst || it >= max_iters || !(worst <= 0.5 * prev)) break;
+    prev = worst;
   }
+  *iters = it;
   // the final iterate into the caller's d (the iterate set is packed like d)
+  if ((*converged || write_unconverged) && write_iterate(m, d, xa, s) != 0) return -1;
+  return 0;
+}
+
+int mixed_measure(MixedState& m, const MixedSolveHooks& h, hipStream_t s, double* worst) {
   size_t nd_all = 0;
   for (int sy = 0; sy < m.nsys; ++sy) nd_all += (size_t)m.Ns[sy] * m.n * m.nrhs;
-  e = hipMemcpyAsync(d, m.sys_host[0].x[xa], nd_all * 8, hipMemcpyDeviceToDevice, s);
+  // a zero FP32 correction in set 0, then x_1 = x_0 + 0 and r = d - M x_0
+  hipError_t e = hipMemsetAsync(m.sys_host[0].v[0], 0, nd_all * 4, s);
+  if (e == hipSuccess) e = residual_prof(m, h, false, 0, 0, 1, 1, m.mnorm_stale ? 1 : 0, s);
+  m.mnorm_stale = false;
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;
   }
-  *iters = it;
-  return 0;
+  return read_berr(m, s, worst);
 }
 
 const char* mixed_last_error() { return g_mixed_err.c_str(); }

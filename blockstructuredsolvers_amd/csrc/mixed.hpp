@@ -70,7 +70,9 @@ struct MixedSolveHooks {
   // optional, called once right after the first solve32 is enqueued, so that the first
   // solve is queued behind earlier work while the caller waits on it: returns 0 when
   // that solve stands, 1 when the caller has changed the solve path (the x = 0 pass and
-  // the first solve are then redone from d), anything else on failure
+  // the first solve are then redone from d), 2 when refinement cannot succeed (an FP32
+  // pivot failed: the loop stops, unconverged, without writing d), anything else on
+  // failure
   int (*prepare)(void* ctx, hipStream_t s) = nullptr;
   // optional profiler brackets around the residual and backward-error launches
   void* (*prof_begin)(void* ctx, int kind, hipStream_t s) = nullptr;
@@ -82,10 +84,23 @@ struct MixedSolveHooks {
 void mixed_after_factor(MixedState& m);
 
 // Refinement loop: x_0 = 0; repeat { dx = M32^{-1} r ; x += dx ; r = d - M x } until
-// max berr <= tol or max_iters solves.  On exit d holds x.  Synchronises the stream
-// once per iteration to test convergence.
+// max berr <= tol (*converged = true), or it stops short: a NaN backward error, a
+// backward error that did not halve since the previous check (stagnation: the FP32
+// factors cannot refine this system further), or max_iters solves.  d receives the
+// last iterate when converged, or when `write_unconverged` is set.  Synchronises the
+// stream once per checked iteration.
 int mixed_solve(MixedState& m, const MixedSolveHooks& h, double* d, double tol, int max_iters,
-                int* iters, hipStream_t s, bool first_pass_done = false);
+                int* iters, bool* converged, bool write_unconverged, hipStream_t s,
+                bool first_pass_done = false);
+
+// Backward error of the FP64 solution held in iterate set 0 (the FP64 fallback's
+// answer): one residual pass with a zero correction, which copies it into set 1 and
+// measures berr per system and column into last_berr.  Returns the largest (NaN if
+// any is NaN) through *worst.  The answer to hand out is then in set 1.
+int mixed_measure(MixedState& m, const MixedSolveHooks& h, hipStream_t s, double* worst);
+
+// Device address of iterate set k (FP64, packed like the caller's d).
+inline double* mixed_iterate(const MixedState& m, int k) { return m.sys_host[0].x[k]; }
 
 // The refinement loop's first (x = 0) residual pass on its own: r = d as FP32 into the
 // level-0 vectors of set 0 and ||d||.  It reads only the caller's FP64 d, A and B, so it

@@ -46,7 +46,7 @@ def _stream_handle(device) -> int:
 
 
 def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                 tol, max_iters, graph=False, pivot_tol=None) -> None:
+                 tol, max_iters, graph=False, pivot_tol=None, fallback=None) -> None:
     if chain_len is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN, float(chain_len))
     if base_max is not None:
@@ -56,8 +56,8 @@ def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, pres
             L.call("bss_set_option", plan, L.BSS_OPT_CHAIN_LEN_LEVEL + k, float(m))
     if factor_waves is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_FACTOR_WAVES, float(factor_waves))
-    if preserve_input:
-        L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0)
+    if preserve_input is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_PRESERVE_INPUT, 1.0 if preserve_input else 0.0)
     if tol is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_TOL, float(tol))
     if max_iters is not None:
@@ -66,6 +66,8 @@ def _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, pres
         L.call("bss_set_option", plan, L.BSS_OPT_GRAPH, 1.0)
     if pivot_tol is not None:
         L.call("bss_set_option", plan, L.BSS_OPT_PIVOT_TOL, float(pivot_tol))
+    if fallback is not None:
+        L.call("bss_set_option", plan, L.BSS_OPT_FALLBACK, 1.0 if fallback else 0.0)
 
 
 def _torch_workspace(plan, device) -> torch.Tensor:
@@ -90,11 +92,12 @@ class BlockTriDiagData_batched:
 
     def __init__(self, N: int, n: int, T, device, nrhs: int, nsys: int, mode: str,
                  schedule: str, chain_len: Optional[int], base_max: Optional[int],
-                 preserve_input: bool = False, tol: Optional[float] = None,
+                 preserve_input: bool = True, tol: Optional[float] = None,
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
                  factor_waves: Optional[int] = None, graph: bool = False,
-                 pivot_tol: Optional[float] = None, torch_workspace: bool = False):
+                 pivot_tol: Optional[float] = None, torch_workspace: bool = False,
+                 fallback: Optional[bool] = None):
         self.n_user = N
         self.n = n
         self.nrhs = nrhs
@@ -111,7 +114,7 @@ class BlockTriDiagData_batched:
                    ctypes.byref(plan))
         self._plan = plan
         _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                     tol, max_iters, graph, pivot_tol)
+                     tol, max_iters, graph, pivot_tol, fallback)
         self.N = self._get(L.BSS_Q_N_USE)
         self.dtype = _torch_dtype(code)
         NU = self.N
@@ -181,6 +184,18 @@ class BlockTriDiagData_batched:
         return self._get(L.BSS_Q_LAST_ITERS)
 
     @property
+    def fallback(self) -> bool:
+        """Mixed mode: the solves since the last factorize went through the plain
+        FP64 fallback (refinement on the FP32 factors stopped short of ``tol``)."""
+        return bool(self._get(L.BSS_Q_FALLBACK))
+
+    @property
+    def converged(self) -> bool:
+        """Mixed mode: the last solve met ``tol`` for every system and column (the
+        status the solve call returned was BSS_OK, not BSS_NOT_CONVERGED)."""
+        return getattr(self, "last_status", L.BSS_OK) == L.BSS_OK
+
+    @property
     def refactored(self) -> int:
         """Mixed mode: separator levels re-factored in FP64 after the last factorize
         (known once a solve or info() has inspected the FP32 pivots)."""
@@ -215,11 +230,12 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
 
     def __init__(self, Ns: List[int], n: int, T, device, nrhs: int, mode: str,
                  schedule: str, chain_len: Optional[int], base_max: Optional[int],
-                 preserve_input: bool = False, tol: Optional[float] = None,
+                 preserve_input: bool = True, tol: Optional[float] = None,
                  max_iters: Optional[int] = None,
                  chain_len_levels: Optional[List[int]] = None,
                  factor_waves: Optional[int] = None, graph: bool = False,
-                 pivot_tol: Optional[float] = None, torch_workspace: bool = False):
+                 pivot_tol: Optional[float] = None, torch_workspace: bool = False,
+                 fallback: Optional[bool] = None):
         Ns = [int(v) for v in Ns]
         self.Ns_user = Ns
         self.n = n
@@ -238,7 +254,7 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
                    sched_code, ctypes.byref(plan))
         self._plan = plan
         _set_options(plan, chain_len, base_max, chain_len_levels, factor_waves, preserve_input,
-                     tol, max_iters, graph, pivot_tol)
+                     tol, max_iters, graph, pivot_tol, fallback)
         nu = (ctypes.c_int * len(Ns))()
         L.call("bss_plan_system_sizes", plan, nu)
         self.N_use = [int(v) for v in nu]
@@ -271,14 +287,15 @@ class BlockTriDiagData_group(BlockTriDiagData_batched):
 def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int = 1,
                        nsys: int = 1, mode: str = "plain", schedule: str = "auto",
                        chain_len: Optional[int] = None,
-                       base_max: Optional[int] = None, preserve_input: bool = False,
+                       base_max: Optional[int] = None, preserve_input: bool = True,
                        tol: Optional[float] = None,
                        max_iters: Optional[int] = None,
                        chain_len_levels: Optional[List[int]] = None,
                        factor_waves: Optional[int] = None,
                        tuned: bool = True, graph: bool = False,
                        pivot_tol: Optional[float] = None,
-                       torch_workspace: bool = False) -> BlockTriDiagData_batched:
+                       torch_workspace: bool = False,
+                       fallback: Optional[bool] = None) -> BlockTriDiagData_batched:
     """``initialize_batched(N, n, T, M)`` (src/gpu_batched.jl:165-182).
 
     ``M`` names the device (``"cuda"``/``"cuda:k"`` or a torch.device).  Extensions
@@ -286,8 +303,11 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     gpu_batched.jl:257), ``nsys`` same-shape systems in one plan, ``mode="mixed"``
     (FP32 factor + FP64 refinement, T must be Float64) and ``schedule="reference"``
     (the reference's level schedule and identity padding to N_use) or ``"auto"``;
-    ``preserve_input`` keeps A and B unchanged in plain mode (factors go to the plan's
-    workspace); ``tol``/``max_iters`` control mixed-mode refinement; ``chain_len``,
+    ``preserve_input`` (default) keeps A and B unchanged in plain mode (factors go to
+    the plan's workspace; ``False`` overwrites them as the reference does, and
+    ``info()`` then gives a failing block that need not be the LAPACK-order first one);
+    ``tol``/``max_iters`` control mixed-mode refinement and ``fallback`` (default on)
+    its plain FP64 fallback when refinement stops short; ``chain_len``,
     ``chain_len_levels``, ``base_max`` and ``factor_waves`` fix the auto schedule,
     otherwise (``tuned=True``) it comes from the autotuner cache when one exists;
     ``graph=True`` replays each launch sequence from a captured hipGraph;
@@ -311,7 +331,7 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
     return BlockTriDiagData_batched(N, n, T, device, nrhs, nsys, mode, schedule, chain_len,
                                     base_max, preserve_input, tol, max_iters,
                                     chain_len_levels, factor_waves, graph, pivot_tol,
-                                    torch_workspace)
+                                    torch_workspace, fallback)
 
 
 class BlockTriDiagData_seq(BlockTriDiagData_batched):
@@ -359,13 +379,14 @@ def initialize_seq(N: int, n: int, T=torch.float64, M="cuda", *,
 def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *,
                              nrhs: int = 1, mode: str = "plain", schedule: str = "auto",
                              chain_len: Optional[int] = None,
-                             base_max: Optional[int] = None, preserve_input: bool = False,
+                             base_max: Optional[int] = None, preserve_input: bool = True,
                              tol: Optional[float] = None, max_iters: Optional[int] = None,
                              chain_len_levels: Optional[List[int]] = None,
                              factor_waves: Optional[int] = None,
                              tuned: bool = True, graph: bool = False,
                              pivot_tol: Optional[float] = None,
-                             torch_workspace: bool = False) -> BlockTriDiagData_group:
+                             torch_workspace: bool = False,
+                             fallback: Optional[bool] = None) -> BlockTriDiagData_group:
     """Uneven batch: like ``initialize_batched`` but system s has ``Ns[s]`` blocks.
     Schedule options apply to every system.  With ``tuned=True`` the autotuner cache is
     used only when it names one configuration for all the sizes in ``Ns``, so each
@@ -387,7 +408,7 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
             factor_waves = cfgs[0].get("factor_waves")
     return BlockTriDiagData_group(list(Ns), n, T, device, nrhs, mode, schedule, chain_len,
                                   base_max, preserve_input, tol, max_iters, chain_len_levels,
-                                  factor_waves, graph, pivot_tol, torch_workspace)
+                                  factor_waves, graph, pivot_tol, torch_workspace, fallback)
 
 
 def factorize(data: BlockTriDiagData_batched) -> None:
@@ -396,19 +417,26 @@ def factorize(data: BlockTriDiagData_batched) -> None:
         L.call("bss_factorize", data._plan, _stream_handle(data.device))
 
 
-def solve(data: BlockTriDiagData_batched) -> None:
-    """``solve!(data)`` (src/gpu_batched.jl:229-259): solution in place in d_tensor."""
+def solve(data: BlockTriDiagData_batched) -> int:
+    """``solve!(data)`` (src/gpu_batched.jl:229-259): solution in place in d_tensor.
+    Returns the status: 0, or (mixed mode) ``BSS_NOT_CONVERGED`` = 1 when some system
+    ended above the backward-error target even after the FP64 fallback; like the
+    reference, nothing is raised for a numerical outcome (``data.converged``)."""
     with torch.cuda.device(data.device):
-        L.call("bss_solve", data._plan, _stream_handle(data.device))
+        rc = L.call("bss_solve", data._plan, _stream_handle(data.device))
+    data.last_status = rc
+    return rc
 
 
-def factorize_solve(data: BlockTriDiagData_batched) -> None:
+def factorize_solve(data: BlockTriDiagData_batched) -> int:
     """``factorize!(data); solve!(data)`` (src/gpu_batched.jl:184-259) in one call, for
-    a d that is final before the factorisation: the first solve's level-0 forward
-    sweep overlaps the separator levels' factorisation on a second stream.  Same
-    results and state as ``factorize`` then ``solve``; asynchronous."""
+    a d that is final before the factorisation: the first solve's forward sweeps
+    overlap the separator levels' factorisation on a second stream.  Same results,
+    state and status as ``factorize`` then ``solve``; asynchronous."""
     with torch.cuda.device(data.device):
-        L.call("bss_factorize_solve", data._plan, _stream_handle(data.device))
+        rc = L.call("bss_factorize_solve", data._plan, _stream_handle(data.device))
+    data.last_status = rc
+    return rc
 
 
 factorize_ = factorize

@@ -18,7 +18,9 @@
  * Status:   every entry point returns 0 on success, a negative BSS_E* code on an
  *           argument or HIP error (message via bss_last_error), and never throws.
  *           Numerical failure (a non-positive pivot) is not an error status: it is
- *           reported per system by bss_info, like LAPACK's info.
+ *           reported per system by bss_info, like LAPACK's info.  A mixed-mode solve
+ *           that ends above its backward-error target returns the positive
+ *           BSS_NOT_CONVERGED instead of BSS_OK.
  * Threads:  calls on one plan must be serialised by the caller; different plans
  *           may be used from different threads.  Work is enqueued on the stream
  *           passed in (NULL = the default stream) and the calls return before it
@@ -52,7 +54,12 @@ enum bss_status {
   BSS_E_HIP = -2,      /* HIP runtime error */
   BSS_E_NOMEM = -3,    /* device allocation failed */
   BSS_E_STATE = -4,    /* call out of order (e.g. solve before factorize) */
-  BSS_E_UNSUPPORTED = -5
+  BSS_E_UNSUPPORTED = -5,
+  /* positive: numerical outcome, not an error.  Mixed mode: the last solve did not
+   * reach the backward-error target, neither by FP32-factor refinement nor by the FP64
+   * fallback (or with the fallback off); d holds the last iterate and
+   * bss_last_backward_error its backward error (NaN for a non-SPD system) */
+  BSS_NOT_CONVERGED = 1
 };
 
 /* Plan queries (bss_plan_get). */
@@ -65,7 +72,9 @@ enum bss_query {
   BSS_Q_LEVEL_CHAINS = 48,  /* + level: chains of all systems (one workgroup each) */
   BSS_Q_LEVEL_MAXLEN = 80,  /* + level: longest chain */
   BSS_Q_LAST_ITERS = 112,   /* refinement iterations used by the last mixed solve */
-  BSS_Q_REFACTORED = 113    /* separator levels re-factored in FP64 (mixed mode) */
+  BSS_Q_REFACTORED = 113,   /* separator levels re-factored in FP64 (mixed mode) */
+  BSS_Q_FALLBACK = 114      /* mixed mode: 1 when the solves since the last factorize
+                               run through the plain FP64 fallback (see BSS_OPT_FALLBACK) */
 };
 
 /* Options (bss_set_option).  Schedule and storage options (CHAIN_LEN*, BASE_MAX,
@@ -80,9 +89,14 @@ enum bss_option {
                                bss_solve or bss_info (which synchronise) */
   BSS_OPT_CHAIN_LEN = 3,    /* auto schedule: chain length for every level */
   BSS_OPT_BASE_MAX = 4,     /* auto schedule: largest single-chain base level */
-  BSS_OPT_PRESERVE_INPUT = 5, /* plain mode: 1 = keep A and B unchanged, factors go to
-                               the plan's workspace (default 0: overwrite, as the
-                               reference does) */
+  BSS_OPT_PRESERVE_INPUT = 5, /* plain mode: 1 (default) = keep A and B unchanged, the
+                               factors go to the plan's workspace (twice the A + B
+                               bytes of workspace); 0 = overwrite A and B with the
+                               factors, as the reference does.  With 0, bss_info still
+                               flags every system that failed, but the index it gives
+                               is a failing block of the solver's own elimination order,
+                               not necessarily the smallest one LAPACK order gives (the
+                               input the sequential re-check needs is gone) */
   BSS_OPT_FACTOR_WAVES = 6,  /* waves per factor workgroup at n = 64: 4 (default) or 8 */
   BSS_OPT_GRAPH = 7,         /* 1 = capture each launch sequence (factorize; solve; in
                                mixed mode each FP32 correction solve) into a hipGraph on
@@ -94,6 +108,14 @@ enum bss_option {
   BSS_OPT_PROFILE_MAX_LEVEL = 9, /* profiler: bracket launches at levels <= value only
                                (default: every level; the residual and backward-error
                                kernels count as level 0) */
+  BSS_OPT_FALLBACK = 10,     /* mixed: 1 (default) = when refinement stops short of the
+                               target (a NaN backward error, a backward error that no
+                               longer halves per iteration, max_iters reached, or an FP32
+                               level-0 pivot failure), factor all systems once in plain
+                               FP64 (input preserved, workspace allocated on first use)
+                               and solve this and every later right-hand side of the
+                               factorisation with it, its backward error measured in
+                               FP64; 0 = return BSS_NOT_CONVERGED with the last iterate */
   BSS_OPT_CHAIN_LEN_LEVEL = 32 /* + level k (< 16): chain length of level k only
                                   (auto schedule); overrides BSS_OPT_CHAIN_LEN there */
 };
@@ -155,9 +177,10 @@ int bss_plan_set_workspace(bss_plan_t plan, void* ws, size_t bytes);
 
 /* Bind the caller's device buffers (reference tensors A_tensor, B_tensor, d_tensor
  * of BlockTriDiagData_batched, src/gpu_batched.jl:14-27).  A and B are read by
- * bss_factorize; in plain mode they are overwritten with the factors (as the
- * reference overwrites them); in mixed mode they are left unchanged (the FP64
- * residual needs them).  d holds the right-hand sides and receives the solution. */
+ * bss_factorize and left unchanged (the mixed mode's FP64 residual and bss_info's
+ * sequential re-check read them again); in plain mode with BSS_OPT_PRESERVE_INPUT = 0
+ * they are overwritten with the factors, as the reference overwrites them.  d holds
+ * the right-hand sides and receives the solution. */
 int bss_plan_bind(bss_plan_t plan, void* A, void* B, void* d);
 
 /* Factorize all systems.  Replaces factorize!(::BlockTriDiagData_batched)
@@ -170,15 +193,19 @@ int bss_factorize(bss_plan_t plan, void* stream);
  * Replaces solve!(::BlockTriDiagData_batched) (src/gpu_batched.jl:229-259) and
  * compute_schur_RHS!, update_boundary!, cholesky_solve_batched! with nd = 1
  * (src/backbone_rocm.jl:84-170).  In mixed mode this runs FP32 solves inside FP64
- * iterative refinement and synchronises the stream once per iteration. */
+ * iterative refinement and synchronises the stream once per iteration; it returns
+ * BSS_OK when every system and column reached the target (by refinement, or by the
+ * FP64 fallback of BSS_OPT_FALLBACK), else BSS_NOT_CONVERGED. */
 int bss_solve(bss_plan_t plan, void* stream);
 
 /* bss_factorize followed by bss_solve, with the same results and the same state
  * afterwards (later bss_solve calls reuse the factors), for a caller whose d is
- * final before the factorisation starts.  The first solve's level-0 forward sweep
- * (and, in mixed mode, its x = 0 residual pass) runs on a plan-owned stream beside
- * the factorisation of the separator levels, and is joined back into `stream`
- * before the call returns its work.  Replaces the reference's back-to-back
+ * final before the factorisation starts.  Levels 0 and 1 are factored on `stream`;
+ * then the first solve's forward sweeps (and, in mixed mode, its x = 0 residual pass)
+ * run on a plan-owned stream: those of levels 0 and 1 beside the factorisation of
+ * level 2, and each later level's as soon as that level is factored.  The side stream
+ * is joined back into `stream` before the backward sweeps.  Returns what bss_solve
+ * returns.  Replaces the reference's back-to-back
  * factorize!(data); solve!(data) (src/gpu_batched.jl:184-259, as called in
  * test/test_rocm.jl:97-101); the reference has no combined call.  With BSS_OPT_GRAPH on,
  * or a single level, it is exactly bss_factorize then bss_solve. */
@@ -197,9 +224,15 @@ int bss_build_smoother(int N, int n, int m, const double* F, const double* H,
                        void* stream);
 
 /* Per-system numerical status after the last factorize (synchronises the stream):
- * info[s] = 0 if system s factored, else the smallest 1-based block index whose
- * pivot was non-positive.  The reference passes an `info` buffer to potrf and never
- * reads it (src/backbone_rocm.jl:64-80); this exposes it. */
+ * info[s] = 0 if system s is positive definite, else the 1-based index of the block
+ * at which the sequential block Cholesky of src/backbone_cpu.jl:5-15 (blocks 1, 2, ...
+ * in order, LAPACK potrf on each Schur-updated diagonal block) meets its first
+ * non-positive pivot -- the LAPACK-order answer, whatever the recursive elimination
+ * order met first.  A system flagged by the recursive factorisation is re-factored
+ * once, as a single chain in that order, from the preserved input (in FP64 for mixed
+ * plans, whose info therefore describes the FP64 matrix); see BSS_OPT_PRESERVE_INPUT
+ * for the one case without preserved input.  The reference passes an `info` buffer to
+ * potrf and never reads it (src/backbone_rocm.jl:64-80); this exposes it. */
 int bss_info(bss_plan_t plan, int* info_host);
 
 /* Per-system, per-column backward error of the last mixed-mode solve
