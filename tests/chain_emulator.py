@@ -239,15 +239,16 @@ def plant_positions(N, n, schedule=0):
 
 
 def coupled_failure(sysm, k):
-    """A copy of `sysm` whose LAPACK-order failure is block k (0-based, k >= 2) although
-    A_k alone is SPD: A_{k-1} = A_k = I, B_{k-2} = 0 and B_{k-1} = 2 I, so the pair
-    [[I, 2I], [2I, I]] is indefinite and the sequential pivot at k is I - 4 I."""
+    """A copy of `sysm` whose LAPACK-order failure is block k (0-based, k >= 1) although
+    A_k alone is SPD: A_{k-1} = A_k = I, B_{k-2} = 0 (when k >= 2) and B_{k-1} = 2 I, so
+    the pair [[I, 2I], [2I, I]] is indefinite and the sequential pivot at k is I - 4 I."""
     from blockstructuredsolvers_amd import workloads
     n = sysm.A.shape[0]
     A = sysm.A.copy(order="F")
     B = sysm.B.copy(order="F")
     A[:, :, k - 1] = np.eye(n)
     A[:, :, k] = np.eye(n)
-    B[:, :, k - 2] = 0.0
+    if k >= 2:
+        B[:, :, k - 2] = 0.0
     B[:, :, k - 1] = 2.0 * np.eye(n)
     return workloads.BlockSystem(A, B, sysm.d)

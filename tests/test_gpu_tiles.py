@@ -75,7 +75,7 @@ def test_single_chain_factors(n):
     slots hold Uinv_i (upper 16x16 tiles) and the B slots C_i."""
     N = 4
     s = workloads.spd_system(N, n, nrhs=1, seed=n)
-    data = initialize_batched(N, n, torch.float64, "cuda")
+    data = initialize_batched(N, n, torch.float64, "cuda", preserve_input=False)
     data.A_tensor.copy_(torch.from_numpy(s.A))
     data.B_tensor.copy_(torch.from_numpy(s.B))
     factorize(data)
