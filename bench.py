@@ -349,6 +349,66 @@ def alpha_point(alpha, args, torch, bss, oracle):
     return out
 
 
+def batch_line(args, torch, oracle):
+    """Uneven batch (512 systems, N_k log-uniform in [16, 4096], n_k uniform over
+    {16, 32, 48, 64, 96, 128}, generated on the device): one group plan per n, each on
+    its own stream; a step factors and solves every system.  Plain FP64 and mixed.  Two
+    systems per n (the smallest and one of median size up to 2048 blocks) get their
+    backward error from the FP64 oracle, as the checker."""
+    from blockstructuredsolvers_amd import batch_bench
+    Ns, ns = batch_bench.draw(args.batch_systems, 7)
+    groups = [(n, [N for N, m in zip(Ns, ns) if m == n]) for n in batch_bench.SIZES]
+    groups = [(n, g) for n, g in groups if g]
+    pick = []
+    for gi, (n, g) in enumerate(groups):
+        order = np.argsort(g)
+        small = [int(s) for s in order if g[s] <= 2048]
+        pick += [(gi, s) for s in sorted({int(order[0]), small[len(small) // 2]})]
+    out = {"systems": args.batch_systems, "blocks_total": int(sum(Ns)),
+           "systems_per_n": {str(n): len(g) for n, g in groups},
+           "bytes_A_B_fp64": int(sum(8 * n * n * (2 * N - 1) for N, n in zip(Ns, ns)))}
+    for mode in ("plain", "mixed"):
+        plans, ms, samples = batch_bench.run(mode, groups, steps=args.batch_steps, warmup=1,
+                                             seed=7, check=0, picks=pick)
+        worst = 0.0
+        for gi, s, d0 in samples:
+            data = plans[gi]
+            A = np.asfortranarray(data.A_tensors[s].double().cpu().numpy())
+            B = np.asfortranarray(data.B_tensors[s].double().cpu().numpy())
+            x = data.d_tensors[s].double().cpu().numpy()
+            worst = max(worst, float(oracle.backward_error(A, B, x, np.asfortranarray(d0)).max()))
+        out[mode] = {"ms_per_step": ms, "systems_per_s": args.batch_systems / ms * 1e3,
+                     "checked_systems": len(samples),
+                     "max_backward_error_oracle": worst}
+        if mode == "mixed":
+            out[mode]["iterations_per_n"] = {str(n): int(d.iterations)
+                                             for (n, _), d in zip(groups, plans)}
+        for d in plans:
+            d.close()
+        del plans
+        torch.cuda.empty_cache()
+        log(f"bench: batch {mode}: {out[mode]}")
+    out["mixed_over_plain"] = out["mixed"]["systems_per_s"] / out["plain"]["systems_per_s"]
+    return out
+
+
+def serve_lines(args):
+    """Served stream (Poisson arrivals of smoother systems, n in {16, 32, 64}, N ~
+    U[128, 4096], 1-8 right-hand sides per factorisation, blocks built on the device):
+    throughput and p50/p99 latency at a light and a saturating rate, plain and mixed."""
+    from blockstructuredsolvers_amd import serve_bench
+    out = []
+    for mode in ("plain", "mixed"):
+        for rate in args.serve_rates:
+            r = serve_bench.run(rate, args.serve_count, mode, workers=8, seed=0, N_max=4096)
+            keep = {k: r[k] for k in ("mode", "rate_per_s", "requests", "systems_per_s",
+                                      "latency_ms_p50", "latency_ms_p99", "solve_ms_p50",
+                                      "solve_ms_p99", "queue_ms_p50", "failed")}
+            out.append(keep)
+            log(f"bench: served {keep}")
+    return out
+
+
 def load_traffic(kernel, mode, args):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy of known size) for this workload, if recorded."""
@@ -419,6 +479,13 @@ def main():
                     help="conditioning sweep of the mixed path (empty: skip)")
     ap.add_argument("--no-pipelined", action="store_true",
                     help="skip the two-stream pipelined throughput measurement")
+    ap.add_argument("--batch-systems", type=int, default=512,
+                    help="uneven-batch line: systems per step (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-count", type=int, default=128,
+                    help="served-stream lines: requests per rate and mode (0: skip)")
+    ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
+                    default=[200.0, 3200.0])
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -503,6 +570,9 @@ def main():
             sweep.append(alpha_point(a, args, torch, bss, oracle))
             log(f"    {sweep[-1]}")
 
+    batch = batch_line(args, torch, oracle) if args.batch_systems else None
+    served = serve_lines(args) if args.serve_count else None
+
     head = results.get("mixed") or results.get("plain")
     plain = results.get("plain")
     roof = roofline_for(head, args, args.nsys)
@@ -541,6 +611,8 @@ def main():
         "factor_gflops": {m: factor_gflops(r, args) for m, r in results.items()},
         "kernel_rooflines": {m: kernel_rooflines(r, args, args.nsys) for m, r in results.items()},
         "conditioning_sweep": sweep,
+        "uneven_batch": batch,
+        "served_stream": served,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

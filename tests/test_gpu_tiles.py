@@ -91,3 +91,43 @@ def test_single_chain_factors(n):
     for i in range(N - 1):
         err = np.abs(B_out[:, :, i] - C[i]).max() / np.abs(C[i]).max()
         assert err < 1e-12, f"block {i}: C error {err:.3e}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_update_probe_products(dtype):
+    """The update-product probe (the factor kernel's trsm and Schur products on their
+    own, used for their MFMA counters) computes [C|W] = Uinv^T [B|Z], S = W^T W,
+    Zn = -C^T W and A' = A - C^T C (upper tiles of S and A')."""
+    lib = ctypes.CDLL(PROBE)
+    lib.update_probe.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 5
+    lib.update_probe.restype = ctypes.c_int
+    n, nb = 64, 3
+    rng = np.random.default_rng(5)
+    # (block, col, row) storage: tril of the storage is an upper-triangular matrix
+    U = np.tril(rng.standard_normal((nb, n, n)) * 0.1)
+    BZ = rng.standard_normal((nb, 2 * n, n))
+    A = rng.standard_normal((nb, n, n))
+    dev = [torch.from_numpy(x).to(dtype).cuda() for x in (U, BZ, A)]
+    out = torch.zeros((nb, 5 * n, n), dtype=dtype, device="cuda")
+    rc = lib.update_probe(1 if dtype == torch.float64 else 0, nb, 1, dev[0].data_ptr(),
+                          dev[1].data_ptr(), dev[2].data_ptr(), out.data_ptr(),
+                          torch.cuda.current_stream().cuda_stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = out.double().cpu().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    upper = np.kron(np.triu(np.ones((n // 16, n // 16))), np.ones((16, 16))).astype(bool)
+    for b in range(nb):
+        # storage (col, row) -> matrices
+        Um, BZm, Am = U[b].T, BZ[b].T, A[b].T
+        CW = Um.T @ BZm
+        C, W = CW[:, :n], CW[:, n:]
+        o = got[b]
+        scale = np.abs(CW).max()
+        assert np.abs(o[:2 * n].T - CW).max() <= tol * scale * n
+        S = W.T @ W
+        assert np.abs((o[2 * n:3 * n].T - S)[upper]).max() <= tol * np.abs(S).max() * n
+        Zn = -C.T @ W
+        assert np.abs(o[3 * n:4 * n].T - Zn).max() <= tol * np.abs(Zn).max() * n
+        Ap = Am - C.T @ C
+        assert np.abs((o[4 * n:].T - Ap)[upper]).max() <= tol * np.abs(Ap).max() * n
