@@ -16,6 +16,7 @@ BSS_F32, BSS_F64 = 0, 1
 BSS_MODE_PLAIN, BSS_MODE_MIXED = 0, 1
 BSS_SCHED_AUTO, BSS_SCHED_REFERENCE = 0, 1
 BSS_OK, BSS_NOT_CONVERGED = 0, 1
+BSS_E_ARG, BSS_E_HIP, BSS_E_NOMEM, BSS_E_STATE, BSS_E_UNSUPPORTED = -1, -2, -3, -4, -5
 
 BSS_Q_N_USE = 0
 BSS_Q_LEVELS = 1
@@ -58,6 +59,7 @@ SIGNATURES = {
     "bss_factorize": ([_VP, _VP], _I),
     "bss_solve": ([_VP, _VP], _I),
     "bss_factorize_solve": ([_VP, _VP], _I),
+    "bss_factorize_solve_noalloc": ([_I, _I, _I, _I, _VP, _VP, _VP, _VP], _I),
     "bss_info": ([_VP, ctypes.POINTER(_I)], _I),
     "bss_last_backward_error": ([_VP, ctypes.POINTER(_D)], _I),
     "bss_sync": ([_VP], _I),

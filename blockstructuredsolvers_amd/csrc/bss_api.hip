@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -126,6 +127,10 @@ struct bss_plan {
   bool fb_factored = false;  // fb holds the factors of the current A, B
   bool fb_active = false;    // solves of this factorisation go through fb
   bool fp32_failed = false;  // an FP32 level-0 pivot failed in the last factorisation
+  // set-up traffic (workspace clear, table uploads): synchronous on the null stream, or
+  // stream-ordered on setup_stream (the allocation-free entry point)
+  bool setup_on_stream = false;
+  hipStream_t setup_stream = nullptr;
   // mixed-precision state
   bss::MixedState mx;
   size_t mixed_off = kNone;
@@ -413,6 +418,22 @@ bss::LevelSys level_sys(const bss_plan* p, int s, size_t l, int k, bool wide = f
   return t;
 }
 
+// Host -> device copy of plan set-up data: synchronous, or stream-ordered on the plan's
+// set-up stream.  A stream-ordered copy from pageable host memory has taken the source
+// bytes when the call returns, so the host vectors may go out of scope right after.
+hipError_t h2d(const bss_plan* p, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  if (p->setup_on_stream)
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->setup_stream);
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+__global__ void zero_kernel(uint4* __restrict__ p, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 int upload_tables(bss_plan* p, size_t l) {
   GLevel& G = p->gl[l];
   for (int k = 0; k < (l <= 1 ? 2 : 1); ++k) {
@@ -420,8 +441,8 @@ int upload_tables(bss_plan* p, size_t l) {
       if (wide && l == 1 && k == 1) continue;  // FP64 tables vary at level 0 only
       std::vector<bss::LevelSys> tab;
       for (int s : G.slot_sys) tab.push_back(level_sys(p, s, l, k, wide != 0));
-      HIP_TRY(hipMemcpy(wide ? G.dtab64[k] : G.dtab[k], tab.data(),
-                        tab.size() * sizeof(bss::LevelSys), hipMemcpyHostToDevice));
+      HIP_TRY(h2d(p, wide ? G.dtab64[k] : G.dtab[k], tab.data(),
+                  tab.size() * sizeof(bss::LevelSys)));
     }
   }
   return BSS_OK;
@@ -439,10 +460,20 @@ int build(bss_plan* p) {
     }
     p->ws_owned = true;
   }
-  // set-up traffic goes through the null stream and is complete before this returns,
-  // so kernels on any (also non-blocking) stream see it
-  HIP_TRY(hipMemsetAsync(p->ws, 0, p->ws_bytes, nullptr));
-  HIP_TRY(hipStreamSynchronize(nullptr));
+  if (p->setup_on_stream) {
+    // stream-ordered behind the caller's earlier work on the same memory (a fill
+    // kernel rather than a memset node, so a caller may capture it into a graph)
+    const size_t n16 = p->ws_bytes / 16;  // the layout rounds every piece to 256 bytes
+    const unsigned grid = (unsigned)std::min<size_t>((n16 + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, p->setup_stream,
+                       (uint4*)p->ws, n16);
+    HIP_TRY(hipGetLastError());
+  } else {
+    // set-up traffic goes through the null stream and is complete before this returns,
+    // so kernels on any (also non-blocking) stream see it
+    HIP_TRY(hipMemsetAsync(p->ws, 0, p->ws_bytes, nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+  }
   p->dinfo = (int*)(p->ws + p->info_off);
   if (p->mode == BSS_MODE_MIXED) {
     std::vector<int> nu(p->nsys);
@@ -459,12 +490,8 @@ int build(bss_plan* p) {
       if (p->mode == BSS_MODE_MIXED)
         G.dtab64[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab64[l == 0 ? k : 0]);
     }
-    if (!G.chains.empty())
-      HIP_TRY(hipMemcpy(G.dchains, G.chains.data(), G.chains.size() * sizeof(bss::ChainRec),
-                        hipMemcpyHostToDevice));
-    if (!G.seps.empty())
-      HIP_TRY(hipMemcpy(G.dseps, G.seps.data(), G.seps.size() * sizeof(bss::SepRec),
-                        hipMemcpyHostToDevice));
+    HIP_TRY(h2d(p, G.dchains, G.chains.data(), G.chains.size() * sizeof(bss::ChainRec)));
+    HIP_TRY(h2d(p, G.dseps, G.seps.data(), G.seps.size() * sizeof(bss::SepRec)));
   }
   // original (level-0) block index of every block, per system
   for (int s = 0; s < p->nsys; ++s) {
@@ -473,14 +500,12 @@ int build(bss_plan* p) {
     for (size_t i = 0; i < orig.size(); ++i) orig[i] = (int)i;
     for (size_t l = 0; l < S.lv.size(); ++l) {
       SysLevel& V = S.lv[l];
-      HIP_TRY(hipMemcpy(p->ws + V.oOrig, orig.data(), orig.size() * sizeof(int),
-                        hipMemcpyHostToDevice));
+      HIP_TRY(h2d(p, p->ws + V.oOrig, orig.data(), orig.size() * sizeof(int)));
       std::vector<int> next(V.P);
       for (int k = 0; k < V.P; ++k) next[k] = orig[V.shape.seppos[k]];
       orig.swap(next);
       if (l + 1 < S.lv.size() && V.P > 0)  // next level's block q sits at seppos[q] here
-        HIP_TRY(hipMemcpy(p->ws + S.lv[l + 1].oPrev, V.shape.seppos.data(), V.P * sizeof(int),
-                          hipMemcpyHostToDevice));
+        HIP_TRY(h2d(p, p->ws + S.lv[l + 1].oPrev, V.shape.seppos.data(), V.P * sizeof(int)));
     }
   }
   if (p->mode == BSS_MODE_MIXED) {
@@ -895,6 +920,39 @@ namespace bss {
 int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 }  // namespace bss
 
+// Workspace of bss_factorize_solve_noalloc: a file-scope device array, one copy on every
+// GPU the library runs on, held until the process ends.  In host code its name is not a
+// device address; each device's copy is located once with hipGetSymbolAddress and only
+// that address is used (kernel arguments, copies).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (its address looked up on first use there).
+int scratch_of_current_device(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(BSS_E_UNSUPPORTED, "device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(BSS_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return BSS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int bss_last_error(char* buf, size_t len) {
@@ -1208,6 +1266,49 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
     rs = f64 ? solve_levels<double>(p, 0, s, L) : solve_levels<float>(p, 0, s, L);
   p->fwd_done = 0;
   return rs;
+}
+
+int bss_factorize_solve_noalloc(int N, int n, int nrhs, int dtype, const void* A, const void* B,
+                                void* d, void* stream) {
+  if (N < 1) return fail(BSS_E_ARG, "N must be >= 1 (got %d)", N);
+  if (!A || !d || (N > 1 && !B)) return fail(BSS_E_ARG, "null buffer");
+  char* base = nullptr;
+  if (int e = scratch_of_current_device(&base)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  // a plan on the stack: schedule and layout on the host, every device byte in scratch
+  bss_plan P;
+  P.Ns.assign(1, N);
+  P.n = n;
+  P.nrhs = nrhs;
+  P.nsys = 1;
+  P.dtype = dtype;
+  P.mode = BSS_MODE_PLAIN;
+  P.schedule = BSS_SCHED_AUTO;
+  if (n != 16 && n != 32 && n != 48 && n != 64 && n != 96 && n != 128)
+    return fail(BSS_E_UNSUPPORTED, "block size n=%d not supported (16, 32, 48, 64, 96, 128)", n);
+  if (nrhs < 1 || nrhs > 16) return fail(BSS_E_UNSUPPORTED, "nrhs=%d not in [1, 16]", nrhs);
+  if (dtype != BSS_F32 && dtype != BSS_F64) return fail(BSS_E_ARG, "bad dtype %d", dtype);
+  if (hipGetDevice(&P.device) != hipSuccess) P.device = 0;
+  if (int e = layout(&P)) return e;
+  if (P.ws_bytes > kScratchBytes)
+    return fail(BSS_E_ARG,
+                "N=%d, n=%d, nrhs=%d needs %zu bytes of workspace; the allocation-free entry "
+                "point has %zu (use a plan)", N, n, nrhs, P.ws_bytes, kScratchBytes);
+  P.ws = base;
+  P.ws_owned = false;
+  P.setup_on_stream = true;
+  P.setup_stream = s;
+  P.A = (void*)A;
+  P.B = (void*)B;
+  P.d = d;
+  P.last_stream = s;
+  if (int e = build(&P)) return e;  // clears scratch, uploads tables of levels >= 1
+  if (int e = upload_tables(&P, 0)) return e;
+  int rc = dtype == BSS_F64 ? factor_levels<double, double>(&P, s)
+                            : factor_levels<float, float>(&P, s);
+  if (rc == BSS_OK)
+    rc = dtype == BSS_F64 ? solve_levels<double>(&P, 0, s) : solve_levels<float>(&P, 0, s);
+  return rc;
 }
 
 int bss_info(bss_plan_t p, int* info_host) {

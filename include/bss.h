@@ -211,6 +211,22 @@ int bss_solve(bss_plan_t plan, void* stream);
  * or a single level, it is exactly bss_factorize then bss_solve. */
 int bss_factorize_solve(bss_plan_t plan, void* stream);
 
+/* Factorize and solve one system in plain mode (dtype BSS_F64 or BSS_F32; A and B left
+ * unchanged, solution in place in d) without allocating: no hipMalloc/hipFree and no
+ * plan.  The same kernels, schedule (auto, chain length 3, base 8, 4 factor waves) and
+ * results, bit for bit, as bss_factorize + bss_solve on a plan created with those
+ * defaults.  Workspace and device tables come from a 64 MiB device array of the library
+ * (one per GPU, held until the process ends); systems whose workspace exceeds it (FP64:
+ * about 550 blocks at n = 64, 2,200 at n = 32; BSS_Q_WORKSPACE_BYTES of a plan tells)
+ * get BSS_E_ARG.  Set-up copies are ordered on `stream` (from pageable host memory,
+ * so the call waits for the stream's earlier work).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU -- on different streams or threads -- race.  No
+ * per-system info (use a plan and bss_info for the failing block).  No counterpart in
+ * the reference: for callers that cannot allocate device memory (tight serving loops,
+ * code that forbids allocation). */
+int bss_factorize_solve_noalloc(int N, int n, int nrhs, int dtype, const void* A,
+                                const void* B, void* d, void* stream);
+
 /* Build the block-tridiagonal normal equations of a linear state-space smoother on the
  * device: for the model x_{i+1} = F x_i + w (w ~ N(0, Q)), y_i = H x_i + v (v ~ N(0, R)),
  * A_i = Q^{-1} + F^T Q^{-1} F + H^T R^{-1} H (all N blocks) and B_i = -Q^{-1} F (N-1
