@@ -387,6 +387,23 @@ def batch_line(args, torch, oracle):
             d.close()
         del plans
         torch.cuda.empty_cache()
+        # the same batch through one uneven-batch object: one call (one graph launch for
+        # the plain path) per step for all block sizes
+        data, ms_u, samples_u = batch_bench.run_unified(mode, groups, steps=args.batch_steps,
+                                                        warmup=1, seed=7, picks=pick)
+        worst_u = 0.0
+        for s, d0 in samples_u:
+            A = np.asfortranarray(data.A_tensors[s].double().cpu().numpy())
+            B = np.asfortranarray(data.B_tensors[s].double().cpu().numpy())
+            x = data.d_tensors[s].double().cpu().numpy()
+            worst_u = max(worst_u,
+                          float(oracle.backward_error(A, B, x, np.asfortranarray(d0)).max()))
+        out[mode]["one_batch_call"] = {"ms_per_step": ms_u,
+                                       "systems_per_s": args.batch_systems / ms_u * 1e3,
+                                       "max_backward_error_oracle": worst_u}
+        data.close()
+        del data
+        torch.cuda.empty_cache()
         log(f"bench: batch {mode}: {out[mode]}")
     out["mixed_over_plain"] = out["mixed"]["systems_per_s"] / out["plain"]["systems_per_s"]
     return out
@@ -403,7 +420,8 @@ def serve_lines(args):
             r = serve_bench.run(rate, args.serve_count, mode, workers=8, seed=0, N_max=4096)
             keep = {k: r[k] for k in ("mode", "rate_per_s", "requests", "systems_per_s",
                                       "latency_ms_p50", "latency_ms_p99", "solve_ms_p50",
-                                      "solve_ms_p99", "queue_ms_p50", "failed")}
+                                      "solve_ms_p99", "queue_ms_p50", "failed", "groups",
+                                      "max_group")}
             out.append(keep)
             log(f"bench: served {keep}")
     return out

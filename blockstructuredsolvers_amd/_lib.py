@@ -67,6 +67,19 @@ SIGNATURES = {
     "bss_last_error": ([ctypes.c_char_p, ctypes.c_size_t], _I),
     "bss_reference_schedule": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I], _I),
     "bss_plan_level_chains": ([_VP, _I, ctypes.POINTER(_I), _I], _I),
+    "bss_batch_create": ([_I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I, _I, _I, _I,
+                          ctypes.POINTER(_VP)], _I),
+    "bss_batch_set_option": ([_VP, _I, _D], _I),
+    "bss_batch_layout": ([_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], _I),
+    "bss_batch_get": ([_VP, _I, ctypes.POINTER(ctypes.c_int64)], _I),
+    "bss_batch_bind": ([_VP, _VP, _VP, _VP], _I),
+    "bss_batch_factorize": ([_VP, _VP], _I),
+    "bss_batch_solve": ([_VP, _VP], _I),
+    "bss_batch_info": ([_VP, ctypes.POINTER(_I)], _I),
+    "bss_batch_last_backward_error": ([_VP, ctypes.POINTER(_D)], _I),
+    "bss_batch_sync": ([_VP], _I),
+    "bss_batch_destroy": ([_VP], _I),
     "bss_profile_enable": ([_VP, _I], _I),
     "bss_profile_read": ([_VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
 }

@@ -89,6 +89,54 @@ def run(mode: str, groups, steps: int, warmup: int, seed: int, check: int, picks
     return plans, (time.perf_counter() - t0) / steps * 1e3, samples
 
 
+def run_unified(mode: str, groups, steps: int, warmup: int, seed: int, picks=None,
+                graph: bool = True):
+    """The same batch as `run` through one uneven-batch object (bss_batch_*): one call
+    per step for all block sizes, each call one hipGraph launch (plain) or one graph
+    launch for the factorisation plus each group's refinement on its own thread
+    (mixed).  The systems get the same seeded blocks as in `run`, so plain-mode answers
+    are bitwise those of `run`.  Returns (data, ms per step, sampled right-hand sides)."""
+    from . import initialize_batch
+    Ns = [N for _, g in groups for N in g]
+    ns = [n for n, g in groups for _ in g]
+    data = initialize_batch(Ns, ns, torch.float64, "cuda", mode=mode, graph=graph)
+    first = np.cumsum([0] + [len(g) for _, g in groups])
+    for gi, (n, g) in enumerate(groups):
+        gen = torch.Generator(device=data.device).manual_seed(seed + gi)
+        for k, N in enumerate(g):
+            s = int(first[gi]) + k
+            G = torch.randn((N, n, n), generator=gen, device=data.device, dtype=torch.float64)
+            A = torch.bmm(G, G.transpose(1, 2).contiguous())
+            A += n * torch.eye(n, device=data.device, dtype=torch.float64)
+            data.A_tensors[s].copy_(A.permute(1, 2, 0))
+            if N > 1:
+                B = torch.randn((N - 1, n, n), generator=gen, device=data.device,
+                                dtype=torch.float64)
+                data.B_tensors[s].copy_(B.permute(1, 2, 0))
+            d = torch.rand((N, 1, n), generator=gen, device=data.device, dtype=torch.float64)
+            data.d_tensors[s].copy_(d.permute(2, 1, 0))
+    samples = []
+    for gi, k in (picks or []):
+        s = int(first[gi]) + k
+        samples.append((s, data.d_tensors[s].double().cpu().numpy()))
+    d0 = data._d.clone()
+    torch.cuda.synchronize()
+
+    def step():
+        data._d.copy_(d0)
+        factorize(data)
+        solve(data)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return data, (time.perf_counter() - t0) / steps * 1e3, samples
+
+
 def backward_error(A, B, x, d):
     """max over columns of ||d - M x||_inf / (||M||_inf ||x||_inf + ||d||_inf) in FP64,
     M = tridiag(B^T, A, B), blocks in the views' (row, col, block) layout."""

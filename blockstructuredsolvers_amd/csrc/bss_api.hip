@@ -195,6 +195,13 @@ struct bss_plan {
   }
 };
 
+namespace bss {
+int plan_factor_prepare(bss_plan* p, void* stream);
+int plan_factor_kernels(bss_plan* p, hipStream_t c);
+int plan_factor_finish(bss_plan* p, void* stream);
+int plan_solve_kernels(bss_plan* p, hipStream_t c);
+}  // namespace bss
+
 namespace {
 
 size_t fsize(const bss_plan* p) {  // factor element size
@@ -918,6 +925,46 @@ int sequential_info(bss_plan* p, int s, hipStream_t st, int* out) {
 
 namespace bss {
 int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
+// The pieces of bss_factorize / bss_solve, for callers inside the library that enqueue
+// several plans' kernels into one graph (batch.hip): host-side state before the
+// kernels, the kernel sequence itself (no host copies, so it can be captured), and the
+// host-visible bookkeeping after it.
+int plan_factor_prepare(bss_plan* p, void* stream) {
+  if (int e = check_plan(p)) return e;
+  if (!p->A) return fail(BSS_E_STATE, "bss_plan_bind must precede bss_factorize");
+  p->last_stream = (hipStream_t)stream;
+  p->info_valid = false;
+  p->fb_active = p->fb_factored = false;
+  return BSS_OK;
+}
+
+int plan_factor_kernels(bss_plan* p, hipStream_t c) {
+  if (p->mode == BSS_MODE_MIXED) return factor_levels<double, float>(p, c);
+  if (p->dtype == BSS_F64) return factor_levels<double, double>(p, c);
+  return factor_levels<float, float>(p, c);
+}
+
+int plan_factor_finish(bss_plan* p, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (p->mode == BSS_MODE_MIXED) {
+    // FP32 info words and pivot ratios to pinned memory; read at the first solve
+    HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo, sizeof(int) * 3 * p->nsys,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(p->piv_event, s));
+    bss::mixed_after_factor(p->mx);
+    p->pivots_pending = true;
+    p->sep64 = false;
+  }
+  p->factored = true;
+  return BSS_OK;
+}
+
+// plain mode: the solve's kernel sequence
+int plan_solve_kernels(bss_plan* p, hipStream_t c) {
+  if (p->dtype == BSS_F64) return solve_levels<double>(p, 0, c);
+  return solve_levels<float>(p, 0, c);
+}
 }  // namespace bss
 
 // Workspace of bss_factorize_solve_noalloc: a file-scope device array, one copy on every
@@ -1125,33 +1172,12 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
 }
 
 int bss_factorize(bss_plan_t p, void* stream) {
-  if (int e = check_plan(p)) return e;
-  if (!p->A) return fail(BSS_E_STATE, "bss_plan_bind must precede bss_factorize");
+  if (int e = bss::plan_factor_prepare(p, stream)) return e;
   hipStream_t s = (hipStream_t)stream;
-  p->last_stream = s;
-  p->info_valid = false;
-  p->fb_active = p->fb_factored = false;
-  int rc;
-  if (p->mode == BSS_MODE_MIXED) {
-    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, float>(p, c); });
-    if (rc == BSS_OK) {
-      // FP32 info words and pivot ratios to pinned memory; read at the first solve
-      HIP_TRY(hipMemcpyAsync(p->host_piv, p->dinfo, sizeof(int) * 3 * p->nsys,
-                             hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipEventRecord(p->piv_event, s));
-    }
-    if (rc == BSS_OK) {
-      bss::mixed_after_factor(p->mx);
-      p->pivots_pending = true;
-      p->sep64 = false;
-    }
-  } else if (p->dtype == BSS_F64) {
-    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<double, double>(p, c); });
-  } else {
-    rc = run_graph(p, &p->g_factor, s, [p](hipStream_t c) { return factor_levels<float, float>(p, c); });
-  }
-  if (rc == BSS_OK) p->factored = true;
-  return rc;
+  const int rc = run_graph(p, &p->g_factor, s,
+                           [p](hipStream_t c) { return bss::plan_factor_kernels(p, c); });
+  if (rc != BSS_OK) return rc;
+  return bss::plan_factor_finish(p, stream);
 }
 
 int bss_solve(bss_plan_t p, void* stream) {
@@ -1161,9 +1187,8 @@ int bss_solve(bss_plan_t p, void* stream) {
   p->last_stream = s;
   p->fwd_done = 0;
   if (p->mode == BSS_MODE_MIXED) return run_mixed_solve(p, s, false);
-  if (p->dtype == BSS_F64)
-    return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<double>(p, 0, c); });
-  return run_graph(p, &p->g_solve[0], s, [p](hipStream_t c) { return solve_levels<float>(p, 0, c); });
+  return run_graph(p, &p->g_solve[0], s,
+                   [p](hipStream_t c) { return bss::plan_solve_kernels(p, c); });
 }
 
 int bss_factorize_solve(bss_plan_t p, void* stream) {

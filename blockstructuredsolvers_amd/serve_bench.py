@@ -20,11 +20,12 @@ from . import serving
 
 
 def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
-        device_blocks: bool = True) -> dict:
+        device_blocks: bool = True, max_group: int = 64) -> dict:
     reqs = serving.smoother_requests(count, rate, N_range=(128, N_max), seed=seed,
                                      device_blocks=device_blocks)
-    srv = serving.Server(mode=mode, workers=workers)
+    srv = serving.Server(mode=mode, workers=workers, max_group=max_group)
     srv.serve(reqs[:min(4, count)], realtime=False)  # warm-up: kernels loaded, pools primed
+    srv.stats.update(groups=0, max_group=0, max_inflight=0, max_inflight_bytes=0)
     t0 = time.perf_counter()
     res = srv.serve(reqs)
     wall = time.perf_counter() - t0
@@ -32,6 +33,7 @@ def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
     out.update(rate_per_s=rate, mode=mode, workers=workers, wall_s=wall,
                blocks="device-built from the model" if device_blocks else "uploaded",
                offered_span_s=reqs[-1].arrival, max_inflight=srv.stats["max_inflight"],
+               groups=srv.stats["groups"], max_group=srv.stats["max_group"],
                max_inflight_gb=srv.stats["max_inflight_bytes"] / 1e9,
                rhs_total=sum(len(r.rhs) for r in reqs), failed=sum(r.info != 0 for r in res))
     return out
@@ -45,6 +47,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--N-max", type=int, default=4096)
+    ap.add_argument("--max-group", type=int, default=64,
+                    help="requests coalesced into one group plan at most (1: none)")
     ap.add_argument("--upload-blocks", action="store_true",
                     help="send dense A, B with each request instead of the model")
     a = ap.parse_args(argv)
@@ -53,7 +57,7 @@ def main(argv=None) -> int:
         return 2
     for r in [float(x) for x in a.rates.split(",")]:
         print(json.dumps(run(r, a.count, a.mode, a.workers, a.seed, a.N_max,
-                             not a.upload_blocks)), flush=True)
+                             not a.upload_blocks, a.max_group)), flush=True)
     return 0
 
 

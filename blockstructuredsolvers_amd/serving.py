@@ -13,17 +13,26 @@ around the same plans:
   plan workspace + A, B, d, as the plan reports it before any allocation -- fits in the
   HBM budget beside the requests in flight.  There is no batch boundary: a request
   that arrives while others run starts as soon as a slot and memory free up.
-* Each worker thread owns one HIP stream and runs its request end to end: upload,
-  ``factorize``, then for each right-hand side upload, ``solve``, download.  The
-  factors stay resident between the solves.  Work of different workers overlaps on
-  the device (ctypes releases the GIL during library calls).  Plans take their
-  workspace from torch's caching allocator, so admission and release make no device
-  allocation or free.
-* Latency of a request = host time its last solution is available - arrival time.
+* Requests are coalesced: when a worker is free, the FIFO head is admitted together
+  with every other waiting request of the same block size n and right-hand-side width
+  (up to ``max_group`` requests and the HBM budget), and the group runs as one group
+  plan (``bss_plan_create_group``), so its systems share every launch.
+* Each worker thread owns one HIP stream and runs its group end to end: blocks
+  uploaded (or built on the device from the model), one ``factorize`` of all systems,
+  then solve rounds: round k solves the k-th right-hand side of every request that has
+  one (one ``solve`` over the group) and downloads those solutions.  The factors stay
+  resident between the rounds.  Work of different workers overlaps on the device
+  (ctypes releases the GIL during library calls).  Plans take their workspace from
+  torch's caching allocator, so admission and release make no device allocation or
+  free.
+* Latency of a request = host time its last solution is available - arrival time; a
+  solve's latency is its round's device time.
 
-A request's arithmetic depends only on its own (N, n, type, mode) plan, never on what
-else is in flight, so a served answer equals the answer of the same request solved
-alone (bitwise for plain plans).
+A system's arithmetic in a group plan is the arithmetic it gets alone (its own level
+schedule, separate launch slots, no cross-system reductions), so a served answer equals
+the answer of the same request solved alone (bitwise for plain plans; within the
+refinement tolerance in mixed mode, where a group refines until all its systems meet
+the target).
 """
 from __future__ import annotations
 
@@ -39,7 +48,8 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .solver import _dtype_code, build_smoother, factorize, initialize_batched, solve
+from .solver import (_dtype_code, build_smoother, factorize, initialize_batched,
+                     initialize_batched_group, solve)
 
 
 @dataclass
@@ -118,6 +128,47 @@ def request_footprint(N: int, n: int, nrhs: int, dtype=torch.float64, mode: str 
     return ws + 256 + es * n * (2 * N * n + N * nrhs)
 
 
+def _storage_view(t: torch.Tensor) -> torch.Tensor:
+    """The (blocks, cols, rows) C-order storage behind a per-system (rows, cols, blocks)
+    view of a plan tensor."""
+    return t.permute(2, 1, 0)
+
+
+def _build_smoother_into(data, A: torch.Tensor, B: torch.Tensor, req: Request) -> None:
+    """bss_build_smoother into one system's A, B views of a group plan, on the current
+    stream; the model's device copies stay referenced by `data` until it is closed."""
+    n = req.n
+    dev = data.device
+    F, H, Qi, Ri = [torch.from_numpy(np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)).to(dev)
+                    for M in (req.model.F, req.model.H, req.model.Qinv, req.model.Rinv)]
+    code = L.BSS_F64 if data.dtype == torch.float64 else L.BSS_F32
+    L.call("bss_build_smoother", req.N, n, req.model.H.shape[0], F.data_ptr(), H.data_ptr(),
+           Qi.data_ptr(), Ri.data_ptr(), A.data_ptr(), B.data_ptr() if req.N > 1 else None,
+           code, torch.cuda.current_stream(dev).cuda_stream)
+    keep = getattr(data, "_model_tensors", [])
+    keep.append((F, H, Qi, Ri))
+    data._model_tensors = keep
+
+
+def group_footprint(Ns: List[int], n: int, nrhs: int, dtype=torch.float64,
+                    mode: str = "plain") -> int:
+    """Device bytes a coalesced group holds while in flight: its group plan's workspace
+    (host-only query) plus the systems' A, B and d."""
+    mcode = L.BSS_MODE_MIXED if mode == "mixed" else L.BSS_MODE_PLAIN
+    p = ctypes.c_void_p()
+    arr = (ctypes.c_int * len(Ns))(*Ns)
+    L.call("bss_plan_create_group", len(Ns), arr, n, nrhs, _dtype_code(dtype), mcode,
+           L.BSS_SCHED_AUTO, ctypes.byref(p))
+    try:
+        v = ctypes.c_int64()
+        L.call("bss_plan_get", p, L.BSS_Q_WORKSPACE_BYTES, ctypes.byref(v))
+        ws = int(v.value)
+    finally:
+        L.lib().bss_plan_destroy(p)
+    es = 8 if dtype == torch.float64 else 4
+    return ws + 256 + sum(es * n * (2 * N * n + N * nrhs) for N in Ns)
+
+
 class _Staged:
     """A request's host data in pinned memory, prepared before it is served (no A, B
     when the device builds the blocks from the request's model)."""
@@ -137,60 +188,83 @@ class Server:
 
     ``mode``/``dtype`` apply to every request; ``hbm_budget`` (bytes) bounds the summed
     footprint of the requests in flight (default: 80% of the device's free memory);
-    ``workers`` is the number of requests in flight at most (one HIP stream each)."""
+    ``workers`` is the number of groups in flight at most (one HIP stream each) and
+    ``max_group`` the most requests coalesced into one group plan (1: no coalescing)."""
 
     def __init__(self, mode: str = "plain", dtype=torch.float64, device="cuda",
-                 hbm_budget: Optional[int] = None, workers: int = 8, **plan_options):
+                 hbm_budget: Optional[int] = None, workers: int = 8, max_group: int = 64,
+                 **plan_options):
         if not torch.cuda.is_available():
             raise RuntimeError("Server needs a HIP device; none is visible")
         self.mode = mode
         self.dtype = dtype
         self.device = torch.device(device)
         self.workers = workers
+        self.max_group = max(1, int(max_group))
         self.plan_options = plan_options
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(0.8 * free)
         self.hbm_budget = int(hbm_budget)
-        self.stats = {"max_inflight_bytes": 0, "max_inflight": 0}
+        self.stats = {"max_inflight_bytes": 0, "max_inflight": 0, "groups": 0,
+                      "max_group": 0}
 
     # -- admission ---------------------------------------------------------------------
     def footprint(self, N: int, n: int, nrhs: int) -> int:
         return request_footprint(N, n, nrhs, self.dtype, self.mode)
 
-    # -- one request on one worker stream ----------------------------------------------
-    def _run(self, req: Request, st: _Staged, stream: torch.cuda.Stream, t0: float,
-             admitted: float) -> Result:
-        N = req.N
+    def _group_fp(self, group: List[Request]) -> int:
+        if len(group) == 1:
+            r = group[0]
+            return self.footprint(r.N, r.n, r.nrhs)
+        return group_footprint([r.N for r in group], group[0].n, group[0].nrhs, self.dtype,
+                               self.mode)
+
+    # -- one coalesced group on one worker stream -----------------------------------
+    def _run(self, reqs: List[Request], staged: List[_Staged], stream: torch.cuda.Stream,
+             t0: float, admitted: float) -> List[Result]:
+        n, nrhs = reqs[0].n, reqs[0].nrhs
         with torch.cuda.stream(stream):
-            data = initialize_batched(N, req.n, self.dtype, self.device, nrhs=req.nrhs,
-                                      mode=self.mode, torch_workspace=True,
-                                      **self.plan_options)
+            data = initialize_batched_group([r.N for r in reqs], n, self.dtype, self.device,
+                                            nrhs=nrhs, mode=self.mode, torch_workspace=True,
+                                            **self.plan_options)
             try:
-                if req.model is not None:
-                    build_smoother(data, req.model)
-                else:
-                    data._a[0].copy_(st.A, non_blocking=True)
-                    if N > 1:
-                        data._b[0].copy_(st.B, non_blocking=True)
+                for k, (r, st) in enumerate(zip(reqs, staged)):
+                    A, B = data.A_tensors[k], data.B_tensors[k]
+                    if r.model is not None:
+                        _build_smoother_into(data, A, B, r)
+                    else:
+                        _storage_view(A).copy_(st.A, non_blocking=True)
+                        if r.N > 1:
+                            _storage_view(B).copy_(st.B, non_blocking=True)
                 factorize(data)
-                xs, solve_ms = [], []
-                for d, out in zip(st.d, st.out):
-                    data._d[0].copy_(d, non_blocking=True)
+                xs = [[] for _ in reqs]
+                solve_ms = [[] for _ in reqs]
+                finished = [0.0] * len(reqs)
+                rounds = max(len(r.rhs) for r in reqs)
+                for j in range(rounds):
+                    live = [k for k, r in enumerate(reqs) if j < len(r.rhs)]
+                    for k in live:
+                        _storage_view(data.d_tensors[k]).copy_(staged[k].d[j], non_blocking=True)
                     e0 = torch.cuda.Event(enable_timing=True)
                     e1 = torch.cuda.Event(enable_timing=True)
                     e0.record(stream)
                     solve(data)
                     e1.record(stream)
-                    out.copy_(data._d[0], non_blocking=True)
+                    for k in live:
+                        staged[k].out[j].copy_(_storage_view(data.d_tensors[k]), non_blocking=True)
                     stream.synchronize()
-                    solve_ms.append(e0.elapsed_time(e1))
-                    xs.append(np.transpose(out.numpy(), (2, 1, 0)).copy())
-                finished = time.perf_counter() - t0
-                info = int(data.info()[0])
+                    ms = e0.elapsed_time(e1)
+                    now = time.perf_counter() - t0
+                    for k in live:
+                        solve_ms[k].append(ms)
+                        xs[k].append(np.transpose(staged[k].out[j].numpy(), (2, 1, 0)).copy())
+                        finished[k] = now
+                info = data.info()
             finally:
                 data.close()
-        return Result(req.rid, xs, req.arrival, admitted, finished, solve_ms, info)
+        return [Result(r.rid, xs[k], r.arrival, admitted, finished[k], solve_ms[k], int(info[k]))
+                for k, r in enumerate(reqs)]
 
     # -- the loop ----------------------------------------------------------------------
     def serve(self, requests: List[Request], realtime: bool = True) -> List[Result]:
@@ -209,17 +283,19 @@ class Server:
         done: "queue.Queue[Tuple[int, int, object]]" = queue.Queue()
         results: Dict[int, Result] = {}
         used = 0
-        inflight: Dict[int, int] = {}   # rid -> footprint
+        inflight: Dict[int, int] = {}   # rid of a group's head -> the group's footprint
+        groups_inflight: Dict[int, List[Request]] = {}
         fifo: List[Request] = []
         nxt = 0
         t0 = time.perf_counter()
 
-        def worker(r: Request, si: int, admitted: float):
+        def worker(group: List[Request], si: int, admitted: float):
             try:
-                res = self._run(r, staged[r.rid], streams[si], t0, admitted)
-                done.put((r.rid, si, res))
+                res = self._run(group, [staged[r.rid] for r in group], streams[si], t0,
+                                admitted)
+                done.put((group, si, res))
             except BaseException as exc:  # reported to the loop, re-raised there
-                done.put((r.rid, si, exc))
+                done.put((group, si, exc))
 
         threads = []
         while len(results) < len(reqs):
@@ -227,20 +303,33 @@ class Server:
             while nxt < len(reqs) and (not realtime or reqs[nxt].arrival <= now):
                 fifo.append(reqs[nxt])
                 nxt += 1
-            # admit in arrival order while a worker and memory are free
+            # admit in arrival order while a worker and memory are free: the FIFO head,
+            # coalesced with the waiting requests of the same shape that still fit
             while fifo and free_streams:
-                r = fifo[0]
-                fp = self.footprint(r.N, r.n, r.nrhs)
-                if used + fp > self.hbm_budget:
+                head = fifo[0]
+                if used + self.footprint(head.N, head.n, head.nrhs) > self.hbm_budget:
                     break
-                fifo.pop(0)
+                group = [head]
+                for r in fifo[1:]:
+                    if len(group) >= self.max_group:
+                        break
+                    if r.n == head.n and r.nrhs == head.nrhs and used + self._group_fp(
+                            group + [r]) <= self.hbm_budget:
+                        group.append(r)
+                fp = self._group_fp(group)
+                ids = {r.rid for r in group}
+                fifo = [r for r in fifo if r.rid not in ids]
                 used += fp
-                inflight[r.rid] = fp
+                inflight[head.rid] = fp
                 self.stats["max_inflight_bytes"] = max(self.stats["max_inflight_bytes"], used)
-                self.stats["max_inflight"] = max(self.stats["max_inflight"], len(inflight))
+                inflight_reqs = sum(len(g) for g in groups_inflight.values()) + len(group)
+                groups_inflight[head.rid] = group
+                self.stats["max_inflight"] = max(self.stats["max_inflight"], inflight_reqs)
+                self.stats["groups"] += 1
+                self.stats["max_group"] = max(self.stats["max_group"], len(group))
                 si = free_streams.pop()
-                th = threading.Thread(target=worker, args=(r, si, time.perf_counter() - t0),
-                                      daemon=True)
+                th = threading.Thread(target=worker,
+                                      args=(group, si, time.perf_counter() - t0), daemon=True)
                 th.start()
                 threads.append(th)
             # completions free their worker and memory
@@ -248,7 +337,7 @@ class Server:
             if nxt < len(reqs) and realtime:
                 timeout = max(0.0, min(timeout, reqs[nxt].arrival - now))
             try:
-                rid, si, res = done.get(timeout=timeout if (fifo or nxt < len(reqs)) else 1.0)
+                group, si, res = done.get(timeout=timeout if (fifo or nxt < len(reqs)) else 1.0)
             except queue.Empty:
                 continue
             while True:
@@ -256,11 +345,13 @@ class Server:
                     for th in threads:
                         th.join()
                     raise res
-                results[rid] = res
-                used -= inflight.pop(rid)
+                for r, out in zip(group, res):
+                    results[r.rid] = out
+                used -= inflight.pop(group[0].rid)
+                groups_inflight.pop(group[0].rid)
                 free_streams.append(si)
                 try:
-                    rid, si, res = done.get_nowait()
+                    group, si, res = done.get_nowait()
                 except queue.Empty:
                     break
         for th in threads:

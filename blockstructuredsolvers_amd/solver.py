@@ -411,10 +411,144 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
                                   factor_waves, graph, pivot_tol, torch_workspace, fallback)
 
 
-def factorize(data: BlockTriDiagData_batched) -> None:
+class BlockTriDiagData_batch:
+    """An uneven batch across block sizes: system s has ``Ns[s]`` blocks of size
+    ``ns[s]``.  The reference's batch is uniform (src/gpu_batched.jl:165-182); here the
+    systems are grouped by n into one group plan each, and by default the whole batch's
+    factorisation (and a plain-mode solve) is one hipGraph launch with the groups as
+    parallel branches (``bss_batch_*``).  Per-system views ``A_tensors[s]`` (n, n,
+    N_use), ``B_tensors[s]``, ``d_tensors[s]`` (n, nrhs, N_use), in input order, over
+    three packed device buffers."""
+
+    def __init__(self, Ns: List[int], ns: List[int], T, device, nrhs: int, mode: str,
+                 schedule: str, graph: bool, tol: Optional[float],
+                 max_iters: Optional[int], preserve_input: bool, fallback: Optional[bool]):
+        if len(Ns) != len(ns) or not Ns:
+            raise ValueError("Ns and ns must name the same, non-zero number of systems")
+        self.Ns_user = [int(v) for v in Ns]
+        self.ns = [int(v) for v in ns]
+        self.nsys = len(Ns)
+        self.nrhs = nrhs
+        self.mode = mode
+        self.device = torch.device(device)
+        code = _dtype_code(T)
+        mode_code = {"plain": L.BSS_MODE_PLAIN, "mixed": L.BSS_MODE_MIXED}[mode]
+        sched_code = {"auto": L.BSS_SCHED_AUTO, "reference": L.BSS_SCHED_REFERENCE}[schedule]
+        b = ctypes.c_void_p()
+        arrN = (ctypes.c_int * self.nsys)(*self.Ns_user)
+        arrn = (ctypes.c_int * self.nsys)(*self.ns)
+        with torch.cuda.device(self.device):
+            L.call("bss_batch_create", self.nsys, arrN, arrn, nrhs, code, mode_code, sched_code,
+                   ctypes.byref(b))
+        self._batch = b
+        opts = [(L.BSS_OPT_GRAPH, 1.0 if graph else 0.0),
+                (L.BSS_OPT_PRESERVE_INPUT, 1.0 if preserve_input else 0.0)]
+        if tol is not None:
+            opts.append((L.BSS_OPT_TOL, float(tol)))
+        if max_iters is not None:
+            opts.append((L.BSS_OPT_MAX_ITERS, float(max_iters)))
+        if fallback is not None:
+            opts.append((L.BSS_OPT_FALLBACK, 1.0 if fallback else 0.0))
+        for opt, v in opts:
+            L.call("bss_batch_set_option", b, opt, v)
+        I64 = ctypes.c_int64
+        oa, ob, od = (I64 * self.nsys)(), (I64 * self.nsys)(), (I64 * self.nsys)()
+        tot = (I64 * 3)()
+        L.call("bss_batch_layout", b, oa, ob, od, tot)
+        self.dtype = _torch_dtype(code)
+        a = torch.zeros(max(int(tot[0]), 1), dtype=self.dtype, device=self.device)
+        bb = torch.zeros(max(int(tot[1]), 1), dtype=self.dtype, device=self.device)
+        dd = torch.zeros(max(int(tot[2]), 1), dtype=self.dtype, device=self.device)
+        self._a, self._b, self._d = a, bb, dd
+        self.A_tensors, self.B_tensors, self.d_tensors, self.N_use = [], [], [], []
+        # N_use per system: the d blocks are packed without gaps, so a system's d range
+        # ends where the next one (in packed order) starts
+        order = sorted(range(self.nsys), key=lambda s: int(od[s]))
+        ends = {}
+        for i, s in enumerate(order):
+            ends[s] = int(od[order[i + 1]]) if i + 1 < len(order) else int(tot[2])
+        for s in range(self.nsys):
+            n = self.ns[s]
+            NU = (ends[s] - int(od[s])) // (n * nrhs)
+            self.N_use.append(NU)
+            A = a[int(oa[s]):int(oa[s]) + NU * n * n].view(NU, n, n).permute(2, 1, 0)
+            nb = max(NU - 1, 0)
+            B = bb[int(ob[s]):int(ob[s]) + nb * n * n].view(nb, n, n).permute(2, 1, 0)
+            d = dd[int(od[s]):int(od[s]) + NU * nrhs * n].view(NU, nrhs, n).permute(2, 1, 0)
+            self.A_tensors.append(A)
+            self.B_tensors.append(B)
+            self.d_tensors.append(d)
+            if NU > self.Ns_user[s]:  # identity padding (gpu_batched.jl:177-179)
+                A[:, :, self.Ns_user[s] - 1:] = torch.eye(n, dtype=self.dtype,
+                                                          device=self.device)[:, :, None]
+        with torch.cuda.device(self.device):
+            L.call("bss_batch_bind", b, a.data_ptr(), bb.data_ptr(), dd.data_ptr())
+
+    def _get(self, q: int) -> int:
+        v = ctypes.c_int64()
+        L.call("bss_batch_get", self._batch, q, ctypes.byref(v))
+        return int(v.value)
+
+    def info(self) -> np.ndarray:
+        out = (ctypes.c_int * self.nsys)()
+        with torch.cuda.device(self.device):
+            L.call("bss_batch_info", self._batch, out)
+        return np.array(out[:], dtype=np.int64)
+
+    def backward_error(self) -> np.ndarray:
+        out = (ctypes.c_double * (self.nsys * self.nrhs))()
+        L.call("bss_batch_last_backward_error", self._batch, out)
+        return np.array(out[:]).reshape(self.nsys, self.nrhs)
+
+    @property
+    def iterations(self) -> int:
+        """Mixed mode: the most refinement iterations any group used in the last solve."""
+        return self._get(L.BSS_Q_LAST_ITERS)
+
+    @property
+    def converged(self) -> bool:
+        return getattr(self, "last_status", L.BSS_OK) == L.BSS_OK
+
+    def synchronize(self) -> None:
+        with torch.cuda.device(self.device):
+            L.call("bss_batch_sync", self._batch)
+
+    def close(self) -> None:
+        if getattr(self, "_batch", None) is not None and self._batch.value:
+            L.lib().bss_batch_destroy(self._batch)
+            self._batch = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def initialize_batch(Ns: List[int], ns: List[int], T=torch.float64, M="cuda", *,
+                     nrhs: int = 1, mode: str = "plain", schedule: str = "auto",
+                     graph: bool = True, tol: Optional[float] = None,
+                     max_iters: Optional[int] = None, preserve_input: bool = True,
+                     fallback: Optional[bool] = None) -> BlockTriDiagData_batch:
+    """Uneven batch across block sizes (``bss_batch_create``): system s has ``Ns[s]``
+    blocks of size ``ns[s]``.  ``factorize``/``solve`` take the returned data like the
+    single-plan data; ``graph`` (default on) runs each call as one hipGraph launch."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("initialize_batch needs a HIP device; none is visible")
+    device = torch.device(M) if not isinstance(M, torch.device) else M
+    if device.type != "cuda":
+        raise ValueError(f"M must name a GPU device, got {M!r}")
+    return BlockTriDiagData_batch(list(Ns), list(ns), T, device, nrhs, mode, schedule, graph,
+                                  tol, max_iters, preserve_input, fallback)
+
+
+def factorize(data) -> None:
     """``factorize!(data)`` (src/gpu_batched.jl:184-227), asynchronous."""
     with torch.cuda.device(data.device):
-        L.call("bss_factorize", data._plan, _stream_handle(data.device))
+        if isinstance(data, BlockTriDiagData_batch):
+            L.call("bss_batch_factorize", data._batch, _stream_handle(data.device))
+        else:
+            L.call("bss_factorize", data._plan, _stream_handle(data.device))
 
 
 def solve(data: BlockTriDiagData_batched) -> int:
@@ -423,7 +557,10 @@ def solve(data: BlockTriDiagData_batched) -> int:
     ended above the backward-error target even after the FP64 fallback; like the
     reference, nothing is raised for a numerical outcome (``data.converged``)."""
     with torch.cuda.device(data.device):
-        rc = L.call("bss_solve", data._plan, _stream_handle(data.device))
+        if isinstance(data, BlockTriDiagData_batch):
+            rc = L.call("bss_batch_solve", data._batch, _stream_handle(data.device))
+        else:
+            rc = L.call("bss_solve", data._plan, _stream_handle(data.device))
     data.last_status = rc
     return rc
 

@@ -37,6 +37,7 @@ extern "C" {
 #endif
 
 typedef struct bss_plan* bss_plan_t;
+typedef struct bss_batch* bss_batch_t;
 
 enum bss_dtype { BSS_F32 = 0, BSS_F64 = 1 };
 enum bss_mode { BSS_MODE_PLAIN = 0, BSS_MODE_MIXED = 1 };
@@ -264,6 +265,39 @@ int bss_plan_destroy(bss_plan_t plan);
 /* Copy the calling thread's last error message into buf (always NUL-terminated);
  * returns the full message length. */
 int bss_last_error(char* buf, size_t len);
+
+/* ---- Uneven batches across block sizes (no counterpart in the reference, whose batch
+ * is uniform: initialize_batched, src/gpu_batched.jl:165-182) ----
+ * `nsys` independent systems, system s with Ns[s] blocks of size ns[s] (any mix of the
+ * supported n), one nrhs, dtype, mode and schedule.  The systems are grouped by n into
+ * one group plan each (bss_plan_create_group: every system keeps the schedule it gets
+ * alone, so its result is bitwise the one it gets alone in plain mode).  With
+ * BSS_OPT_GRAPH on (the default for batches) the whole factorisation -- every level of
+ * every group, the groups as parallel branches -- is one hipGraph launch per call, and
+ * so is a plain-mode solve; a mixed-mode solve runs each group's refinement loop on its
+ * own host thread and stream.  Options (bss_batch_set_option) apply to every group. */
+int bss_batch_create(int nsys, const int* Ns, const int* ns, int nrhs, int dtype, int mode,
+                     int schedule, bss_batch_t* out);
+int bss_batch_set_option(bss_batch_t batch, int option, double value);
+/* Layout of the caller's packed buffers: element offsets of each system's A, B and d
+ * (nsys values each, NULL to skip) and totals[3] = elements of A, B, d.  Groups are
+ * packed in ascending n, each group's systems in input order, each system N_use blocks
+ * in the reference layout.  Returns nsys. */
+int bss_batch_layout(bss_batch_t batch, int64_t* offA, int64_t* offB, int64_t* offd,
+                     int64_t* totals);
+/* bss_plan_get over the groups: the sum for BSS_Q_WORKSPACE_BYTES and BSS_Q_FALLBACK
+ * (groups on the fallback), the largest value for the other queries. */
+int bss_batch_get(bss_batch_t batch, int query, int64_t* value);
+int bss_batch_bind(bss_batch_t batch, void* A, void* B, void* d);
+int bss_batch_factorize(bss_batch_t batch, void* stream);
+/* Returns BSS_OK, or BSS_NOT_CONVERGED when a mixed-mode group did not reach its target. */
+int bss_batch_solve(bss_batch_t batch, void* stream);
+/* Per-system info (input order), as bss_info. */
+int bss_batch_info(bss_batch_t batch, int* info_host);
+/* Mixed mode: per-system, per-column backward error (input order, system-major). */
+int bss_batch_last_backward_error(bss_batch_t batch, double* berr_host);
+int bss_batch_sync(bss_batch_t batch);
+int bss_batch_destroy(bss_batch_t batch);
 
 /* ---- Profiling (no counterpart in the reference, whose benchmarks time whole
  * factorize!/solve! calls with AMDGPU.@elapsed, benchmark/benchmark_gpu.jl:143-150) ----

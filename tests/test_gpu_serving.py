@@ -105,3 +105,22 @@ def test_serve_frames_round_trip():
         alone = serving.solve_alone(r, mode="plain", tuned=False)
         for xs_k, xa in zip(xs, alone):
             assert np.array_equal(xs_k, xa), r.rid
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_coalesced_groups_equal_alone(mode):
+    """Requests waiting together are coalesced into group plans (same n and right-hand-
+    side width; one factorisation and one solve per round for the whole group), and
+    every answer still equals the request solved alone."""
+    reqs = small_stream(16, rate=0.0, seed=31, device_blocks=True)  # all arrive at once
+    srv = serving.Server(mode=mode, workers=2, tuned=False)
+    res = srv.serve(reqs, realtime=False)
+    assert srv.stats["max_group"] >= 3 and srv.stats["groups"] < len(reqs)
+    for r, out in zip(reqs, res):
+        assert out.info == 0 and len(out.x) == len(r.rhs)
+        alone = serving.solve_alone(r, mode=mode, tuned=False)
+        for k, (xs, xa) in enumerate(zip(out.x, alone)):
+            if mode == "plain":
+                assert np.array_equal(xs, xa), (r.rid, k)
+            else:
+                assert np.abs(xs - xa).max() <= 1e-12 * np.abs(xa).max(), (r.rid, k)
