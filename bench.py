@@ -4,8 +4,10 @@ GPU path at normwise backward error <= 1e-12, plus p50/p99 solve latency.
 Workload: block-tridiagonal SPD systems of N=8192 blocks of n=64 (A_i = G G^T + n I,
 B_i ~ N(0,1), d ~ U(0,1), one seed per system), one right-hand side, the size the
 mixed-precision target is quoted on.  A step factorizes and solves `--nsys` resident
-systems (mixed: FP32 MFMA factorisation + FP64 refinement until every system's
-backward error is <= 1e-12; plain: one FP64 factorize + solve).  Inputs are resident
+systems (default 16, where throughput saturates: 2, 8 and 16 systems per step give
+about 1,000, 1,190 and 1,240 mixed systems/s) -- mixed: FP32 MFMA factorisation + FP64
+refinement until every system's backward error is <= 1e-12; plain: one FP64
+factorize + solve.  Both paths use the autotuned schedule for that batch size.  Inputs are resident
 in HBM before timing starts; the plans keep A and B unchanged so every step does the
 full work on the same inputs; each step restores the right-hand side (a 4 MB device
 copy per system) before solving.
@@ -475,7 +477,7 @@ def main():
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--nsys", type=int, default=2)
+    ap.add_argument("--nsys", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--schedule", default="auto", choices=["auto", "reference"])
     ap.add_argument("--chain-len", type=int, default=None)

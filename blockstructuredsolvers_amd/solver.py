@@ -321,9 +321,10 @@ def initialize_batched(N: int, n: int, T=torch.float64, M="cuda", *, nrhs: int =
         raise ValueError(f"M must name a GPU device, got {M!r}")
     if (tuned and schedule == "auto" and chain_len is None and base_max is None
             and chain_len_levels is None and factor_waves is None):
-        # schedule from the autotuner cache (keyed by N, n, type, mode -- never by nsys)
+        # schedule from the autotuner cache (keyed by N, n, type, mode and the largest
+        # tuned batch-size bucket <= nsys)
         from .autotune import lookup
-        cfg = lookup(N, n, T, mode)
+        cfg = lookup(N, n, T, mode, nsys=nsys)
         if cfg:
             chain_len_levels = cfg.get("chain_len_levels")
             base_max = cfg.get("base_max")
@@ -401,7 +402,7 @@ def initialize_batched_group(Ns: List[int], n: int, T=torch.float64, M="cuda", *
     if (tuned and schedule == "auto" and chain_len is None and base_max is None
             and chain_len_levels is None and factor_waves is None):
         from .autotune import lookup
-        cfgs = [lookup(N, n, T, mode) for N in Ns]
+        cfgs = [lookup(N, n, T, mode, nsys=len(Ns)) for N in Ns]
         if cfgs[0] and all(c == cfgs[0] for c in cfgs):
             chain_len_levels = cfgs[0].get("chain_len_levels")
             base_max = cfgs[0].get("base_max")
