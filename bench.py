@@ -492,6 +492,9 @@ def main():
                     help="threads of the CPU baseline (default: every core this process "
                          "may run on)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-direct", type=int, default=1,
+                    help="1: time the CPU direct solvers (LAPACK banded Cholesky, SuperLU) "
+                         "on one assembled system as context")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--modes", default="plain,mixed")
     ap.add_argument("--seed", type=int, default=42)
@@ -573,6 +576,15 @@ def main():
                          f"share {share}) (C restatement of "
                          "cpu.jl, oracle/bss_oracle.c, unblocked loops, no BLAS)"}
 
+    direct = None
+    if not args.no_cpu and args.cpu_direct:
+        # the reference's other CPU comparison (benchmark/benchmark_cpu.jl: sparse direct
+        # solvers on the assembled matrix), with the direct solvers this image has
+        from blockstructuredsolvers_amd import cpu_direct
+        log("bench: CPU direct solvers on the assembled matrix")
+        direct = cpu_direct.run(args.N, args.n, seed=args.seed)
+        log(f"    {direct}")
+
     if args.calib_copy:
         # a plain copy of known size: HBM counters of the kernels above are scaled by
         # what FETCH_SIZE / WRITE_SIZE read for it
@@ -635,6 +647,7 @@ def main():
         "served_stream": served,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_direct_solvers": direct,
     }
     print(json.dumps(out), flush=True)
 
