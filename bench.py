@@ -96,7 +96,8 @@ def run_path(mode, systems, args, torch, bss):
                                   base_max=args.base_max,
                                   tol=args.tol if mode == "mixed" else None,
                                   chain_len_levels=args.chain_lens, factor_waves=args.factor_waves,
-                                  graph=bool(args.graph))
+                                  graph=bool(args.graph),
+                                  pivot_tol=args.pivot_tol if mode == "mixed" else None)
     upload(data, systems, torch)
     d0 = data._d.clone()
     torch.cuda.synchronize()
@@ -486,6 +487,9 @@ def main():
                     help="per-level chain lengths, top level first")
     ap.add_argument("--factor-waves", type=int, default=None)
     ap.add_argument("--graph", type=int, default=0, help="1: replay hipGraph-captured levels")
+    ap.add_argument("--pivot-tol", type=float, default=None,
+                    help="mixed: FP32 separator pivot threshold (2.0 forces the FP64 "
+                         "separator re-factorisation)")
     ap.add_argument("--lat-solves", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0,
