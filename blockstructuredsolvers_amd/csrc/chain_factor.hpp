@@ -47,6 +47,18 @@ struct LdsPad {
 
 __device__ __host__ constexpr int tile_off(int r, int c, int ld) { return 16 * c * ld + 16 * r; }
 
+// The value of x, hidden from the optimiser: offsets derived from it are recomputed
+// where they are used instead of being hoisted out of a kernel's block loop and held in
+// registers for the whole chain.
+__device__ inline int opaque_v(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ inline int opaque_s(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 // Cross-lane helpers for the diagonal-tile routine.
 template <typename T>
 __device__ inline T read_lane(T x, int lane);
@@ -140,9 +152,11 @@ __device__ __attribute__((always_inline)) inline void diag_step(T (&a)[16], T (&
 // Row operations are applied to [A | I]; at the end the right half is E = U^{-T}.
 // Writes Uinv = E^T into the upper triangle of the output tile (strict lower part
 // exact zeros).  Returns false if a pivot was not positive.
-template <typename T>
+// OPQ: the lane's 16 load offsets are recomputed per call, not held across the caller's
+// loop (for kernels whose registers are otherwise exhausted)
+template <typename T, bool OPQ = false>
 __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* at, int lda, T* ut, int ldu) {
-  const int c = lane_id() & 15;
+  const int c = OPQ ? opaque_v(lane_id() & 15) : lane_id() & 15;
   T a[16], e[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -166,14 +180,23 @@ __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* a
 // LA (lookahead): the diagonal step of kk+1 overlaps the elimination of kk (below);
 // without it the diagonal step waits for the whole elimination (the variant with a
 // 128-register cap, where the lookahead's extra live registers spill).
-template <typename T, int NB, int LD, int NW, bool LA>
-__device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf, int* fail) {
+// Slots: waves 1..NW-1 run slot(k), k = 0..TT-1, while wave 0 factors diagonal tile k
+// (LA: slot k >= 1 follows those waves' share of elimination k-1); a barrier separates
+// consecutive slots.  The kernels give them memory work only: an MFMA on the diagonal
+// wave's SIMD stalls its serial VALU steps (a 16x16 diagonal tile measured 2.8K cycles
+// beside idle, VALU, LDS or other-SIMD MFMA work and 6.8K-11.6K beside MFMAs on its
+// own SIMD), and waves w, w + 4 share a SIMD.
+// SL: slots in use (the step loops are unrolled so that each slot's work is fixed at
+// compile time); without it `slot` is never called.
+template <typename T, int NB, int LD, int NW, bool LA, bool SL, typename Slot>
+__device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf, int* fail,
+                                                                Slot slot) {
   constexpr int TT = NB / 16;
   typedef typename Mfma<T>::acc_t acc_t;
   const int w = wave_id();
   auto diag = [&](int kk) {
-    const bool ok = diag_chol_inv16<T>(Abuf + tile_off(kk, kk, LD), LD,
-                                       Ubuf + tile_off(kk, kk, LD), LD);
+    const bool ok = diag_chol_inv16<T, SL>(Abuf + tile_off(kk, kk, LD), LD,
+                                           Ubuf + tile_off(kk, kk, LD), LD);
     if (!ok && lane_id() == 0) *fail = 1;
   };
   // scaling: U_{kk,j} = Uinv_kk^T A_{kk,j} (j > kk);  Et_{t,kk} <- Et_{t,kk} Uinv_kk (t < kk)
@@ -224,8 +247,11 @@ __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf
     }
   };
   if constexpr (!LA) {
-    for (int kk = 0; kk < TT; ++kk) {
-      if (w == 0) diag(kk);
+    auto step = [&](int kk) {
+      if (w == 0)
+        diag(kk);
+      else if (SL)
+        slot(kk);
       __syncthreads();
       scale(kk);
       __syncthreads();
@@ -233,15 +259,24 @@ __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf
         eliminate(kk, w, NW);
         __syncthreads();
       }
+    };
+    if constexpr (SL) {
+#pragma unroll
+      for (int kk = 0; kk < TT; ++kk) step(kk);
+    } else {
+      for (int kk = 0; kk < TT; ++kk) step(kk);
     }
   } else {
     // Lookahead: during the elimination of step kk, wave 0 updates the next diagonal
     // tile (task 0) and factors it at once while the other waves share the remaining
     // tasks, so the serial diagonal step overlaps the elimination instead of following
     // it.  LDS accesses of one wave complete in order.
-    if (w == 0) diag(0);
+    if (w == 0)
+      diag(0);
+    else if (SL)
+      slot(0);
     __syncthreads();
-    for (int kk = 0; kk < TT; ++kk) {
+    auto step = [&](int kk) {
       scale(kk);
       __syncthreads();
       if (kk < TT - 1) {
@@ -254,9 +289,16 @@ __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf
           diag(i);
         } else {
           eliminate(kk, w, NW - 1);
+          if (SL) slot(kk + 1);
         }
         __syncthreads();
       }
+    };
+    if constexpr (SL) {
+#pragma unroll
+      for (int kk = 0; kk < TT; ++kk) step(kk);
+    } else {
+      for (int kk = 0; kk < TT; ++kk) step(kk);
     }
   }
 }
@@ -287,6 +329,38 @@ struct BlockStage {
           s[row * LD + col] = (T)r[q][e];
         else
           s[col * LD + row] = (T)r[q][e];
+      }
+    }
+  }
+};
+
+// One NB x NB block staged through the registers of NTB threads (thread t of them):
+// 16-byte loads issued at once, written to LDS later.
+template <typename Tin, int NB, int NTB>
+struct BgStage {
+  typedef Tin vec_t __attribute__((ext_vector_type(16 / sizeof(Tin))));
+  static constexpr int VEC = 16 / sizeof(Tin);
+  static constexpr int NV = NB * NB / VEC;  // 16-byte vectors per block
+  static constexpr int PER = (NV + NTB - 1) / NTB;
+  vec_t r[PER];
+  __device__ inline void load(const BSS_GLOBAL Tin* g, int t) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int v = q * NTB + t;
+      if (q * NTB + NTB <= NV || v < NV) r[q] = *(const BSS_GLOBAL vec_t*)(g + v * VEC);
+    }
+  }
+  template <typename T, int LD>
+  __device__ inline void store(T* s, int t) const {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int v = q * NTB + t;
+      if (q * NTB + NTB <= NV || v < NV) {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          const int idx = v * VEC + e;
+          s[(idx / NB) * LD + idx % NB] = (T)r[q][e];
+        }
       }
     }
   }
@@ -329,6 +403,15 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr int NS = TT * (TT + 1) / 2;           // upper tiles of a symmetric block
   constexpr int SQ = (NS + NW - 1) / NW;          // S accumulators per wave
   static_assert(NW % TT == 0, "waves must be a multiple of the tile rows");
+  // SLOTS (FP32 factors, n = 64, eight waves, where the 128-register cap leaves no room
+  // to hold a prefetched block through the potrf): the next blocks' HBM loads are
+  // issued and stored by waves 1-7 in the potrf's background slots, while wave 0
+  // factors diagonal tiles (measured 3% faster than loading when needed; the FP64
+  // kernel's prefetch at the top of a block measured faster than slots):
+  //   slot 0: B_blk load;  slot 1: B_blk into the coupling buffer;
+  //   slot 2: A_{blk+1} load, written to the A buffer once the potrf has freed it
+  constexpr bool SLOTS = sizeof(T) == 4 && NB == 64 && NW == 8;
+  constexpr int NTB = SLOTS ? 64 * (NW - 1) : 64;  // slot threads (unused without SLOTS)
   constexpr int XG = NW / TT;                      // wave groups splitting the RHS columns
   constexpr int XQ = 2 * TT / XG;                  // trsm output tiles per wave
   constexpr int ZQ = (TT * TT + NW - 1) / NW;     // panel tiles per wave
@@ -367,7 +450,7 @@ __global__ void __launch_bounds__(64 * NW)
   }
   int fail_i = len;       // first failed block of the chain (thread 0)
   float piv_run = 1.0f;   // smallest pivot ratio of the chain (wave 0)
-  constexpr bool PF = LAT || factor_prefetch<T, NB, NW>();
+  constexpr bool PF = !SLOTS && (LAT || factor_prefetch<T, NB, NW>());
   if (PF && (len > 1 || hasR)) sb.load(B + (long long)start * NN);
   sa.template store<T, LD, false>(Abuf);
   __syncthreads();
@@ -376,10 +459,25 @@ __global__ void __launch_bounds__(64 * NW)
     const int blk = start + i;
     const bool last = (i == len - 1);
     const bool hasC = !last || hasR;  // block couples to a later block
-    if (hasC) {  // B_blk (prefetched, or loaded now)
+    if (hasC && !SLOTS) {  // B_blk (prefetched, or loaded now)
       if (!PF) sb.load(B + (long long)blk * NN);
       sb.template store<T, LD, false>(Rbuf);
     }
+    // B_blk / A_{blk+1} in flight in the slots.  Declared per block: on wave 0's path
+    // they hold no value, so no register is kept for them across its diagonal steps.
+    BgStage<Tin, NB, NTB> gb, ga;
+    auto slot = [&](int k) {
+      if constexpr (SLOTS) {
+        const int bt = opaque_v(tid) - 64;
+        if (k == 0) {
+          if (hasC) gb.load(gptr(B) + (long long)blk * NN, bt);
+        } else if (k == 1) {
+          if (hasC) gb.template store<T, LD>(Rbuf, bt);
+        } else if (k == 2) {
+          if (!last) ga.load(gptr(A) + (long long)(blk + 1) * NN, bt);
+        }
+      }
+    };
     // prefetch the next block pair; it lands while this block is factored
     if (PF && !last) sa.load(A + (long long)(blk + 1) * NN);
     if (PF && (i + 1 < len - 1 || (i + 1 == len - 1 && hasR))) sb.load(B + (long long)(blk + 1) * NN);
@@ -387,7 +485,8 @@ __global__ void __launch_bounds__(64 * NW)
     // pivot monitor: wave 0, lane j < NB keeps the diagonal entry A_jj of this block
     float adiag = 0.0f;
     if (p.pivmin && w == 0 && lane_id() < NB) adiag = (float)Abuf[lane_id() * (LD + 1)];
-    potrf_inv<T, NB, LD, NW, (LAT || factor_min_waves_per_eu<T, NB, NW>() == 1)>(Abuf, Ubuf, &fail);
+    potrf_inv<T, NB, LD, NW, (LAT || factor_min_waves_per_eu<T, NB, NW>() == 1), SLOTS>(
+        Abuf, Ubuf, &fail, slot);
     // failures and pivot ratios are kept per workgroup and reported once after the
     // chain: a global atomic inside the loop, on a path some lanes skip, would make the
     // memory waits after it conservative (and drain the prefetches)
@@ -414,7 +513,8 @@ __global__ void __launch_bounds__(64 * NW)
     // r = (rho - c) mod TT, so every wave does the same amount of MFMA work.
     const int c0 = hasC ? 0 : TT;
     const int c1 = hasL ? 2 * TT : TT;
-    const int xg = w / TT, rho = w % TT;
+    const int wb = SLOTS ? opaque_s(w) : w;  // task tiles recomputed per block
+    const int xg = wb / TT, rho = wb % TT;
     acc_t X[XQ];
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
@@ -434,9 +534,13 @@ __global__ void __launch_bounds__(64 * NW)
       const int r = ((rho - c) % TT + TT) % TT;
       if (c >= c0 && c < c1) tile_store<T>(Rbuf + tile_off(r, c, LD), LD, X[q]);
     }
-    if (!last) {  // A_{blk+1} (prefetched, or loaded now)
-      if (!PF) sa.load(A + (long long)(blk + 1) * NN);
-      sa.template store<T, LD, false>(Abuf);
+    if (!last) {  // A_{blk+1} (prefetched, loaded in a slot, or loaded now)
+      if constexpr (SLOTS) {
+        if (w > 0) ga.template store<T, LD>(Abuf, opaque_v(tid) - 64);
+      } else {
+        if (!PF) sa.load(A + (long long)(blk + 1) * NN);
+        sa.template store<T, LD, false>(Abuf);
+      }
     }
     __syncthreads();
     // factors to HBM as flat 16-byte stores: C (B slot), W (panel slot), Uinv upper tiles
@@ -462,7 +566,7 @@ __global__ void __launch_bounds__(64 * NW)
     if (hasL) {
 #pragma unroll
       for (int q = 0; q < SQ; ++q) {
-        int s = w + NW * q;
+        int s = wb + NW * q;
         if (s < NS) {
           int a = 0;
           while (s >= TT - a) {
@@ -480,7 +584,7 @@ __global__ void __launch_bounds__(64 * NW)
       acc_t Z[ZQ];
 #pragma unroll
       for (int q = 0; q < PQ; ++q) {
-        const int task = w + NW * q;
+        const int task = wb + NW * q;
         if (task < TT * TT) {
           if (hasL && q < ZQ) {
             const int a = task % TT, b = task / TT;
@@ -504,7 +608,7 @@ __global__ void __launch_bounds__(64 * NW)
       if (hasL) {
 #pragma unroll
         for (int q = 0; q < ZQ; ++q) {
-          const int task = w + NW * q;
+          const int task = wb + NW * q;
           if (task < TT * TT) {
             const int a = task % TT, b = task / TT;
             tile_store<T>(Rbuf + tile_off(a, TT + b, LD), LD, Z[q]);
