@@ -178,8 +178,10 @@ __device__ __attribute__((always_inline)) inline bool diag_chol_inv16(const T* a
 // through diag_chol_inv16 and every panel/trailing/inverse update is an MFMA tile
 // product.  On exit Ubuf's upper tiles hold Uinv = U^{-1}.  Abuf is consumed.
 // LA (lookahead): the diagonal step of kk+1 overlaps the elimination of kk (below);
-// without it the diagonal step waits for the whole elimination (the variant with a
-// 128-register cap, where the lookahead's extra live registers spill).
+// without it the diagonal step waits for the whole elimination (the variants with a
+// 128-register cap other than the slot kernel, where the lookahead's extra live
+// registers spill; with the slot kernel's per-call offsets it fits and measured 1.5%
+// faster).
 // Slots: waves 1..NW-1 run slot(k), k = 0..TT-1, while wave 0 factors diagonal tile k
 // (LA: slot k >= 1 follows those waves' share of elimination k-1); a barrier separates
 // consecutive slots.  The kernels give them memory work only: an MFMA on the diagonal
@@ -485,7 +487,7 @@ __global__ void __launch_bounds__(64 * NW)
     // pivot monitor: wave 0, lane j < NB keeps the diagonal entry A_jj of this block
     float adiag = 0.0f;
     if (p.pivmin && w == 0 && lane_id() < NB) adiag = (float)Abuf[lane_id() * (LD + 1)];
-    potrf_inv<T, NB, LD, NW, (LAT || factor_min_waves_per_eu<T, NB, NW>() == 1), SLOTS>(
+    potrf_inv<T, NB, LD, NW, (SLOTS || LAT || factor_min_waves_per_eu<T, NB, NW>() == 1), SLOTS>(
         Abuf, Ubuf, &fail, slot);
     // failures and pivot ratios are kept per workgroup and reported once after the
     // chain: a global atomic inside the loop, on a path some lanes skip, would make the
