@@ -546,8 +546,11 @@ int build(bss_plan* p) {
   return BSS_OK;
 }
 
+// fwd0: level 0's kernel also runs that level's forward sweep of vector set 0 (one
+// right-hand side, bss::factor_fuses_forward)
 template <typename Tin, typename T>
-int factor_levels(bss_plan* p, hipStream_t s, size_t l_begin = 0, size_t l_end = ~size_t(0)) {
+int factor_levels(bss_plan* p, hipStream_t s, size_t l_begin = 0, size_t l_end = ~size_t(0),
+                  bool fwd0 = false) {
   const int nb = p->n;
   const bool mixed = p->mode == BSS_MODE_MIXED;
   if (l_begin == 0) {
@@ -560,9 +563,9 @@ int factor_levels(bss_plan* p, hipStream_t s, size_t l_begin = 0, size_t l_end =
     GLevel& G = p->gl[l];
     // mixed mode: FP32 separator levels report into their own info words and feed the
     // pivot monitor that decides on an FP64 re-factorisation
-    const bss::FactorLevelArgs a{G.dchains, G.dtab[0],
-                                 p->dinfo + ((mixed && l > 0) ? p->nsys : 0),
-                                 (mixed && l > 0) ? p->dpiv : nullptr};
+    bss::FactorLevelArgs a{G.dchains, G.dtab[0], p->dinfo + ((mixed && l > 0) ? p->nsys : 0),
+                           (mixed && l > 0) ? p->dpiv : nullptr};
+    a.fwd = (fwd0 && l == 0) ? 1 : 0;
     hipError_t e;
     {
       ProfScope ps(p, BSS_K_FACTOR, (int)l, s);
@@ -1217,29 +1220,36 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   }
   const bool mixed = p->mode == BSS_MODE_MIXED;
   const bool f64 = !mixed && p->dtype == BSS_F64;
+  // fuse: the level-0 factor kernel runs level 0's forward sweep itself, on the factors
+  // while they are in LDS (one right-hand side; the sweep kernel's arithmetic, so the
+  // same bits), which saves the sweep's pass over the level-0 factors
+  const bool fuse = bss::factor_fuses_forward(p->n, !f64, p->nrhs);
   auto factor = [&](size_t lb, size_t le) {
-    if (mixed) return factor_levels<double, float>(p, s, lb, le);
-    if (f64) return factor_levels<double, double>(p, s, lb, le);
-    return factor_levels<float, float>(p, s, lb, le);
+    if (mixed) return factor_levels<double, float>(p, s, lb, le, fuse);
+    if (f64) return factor_levels<double, double>(p, s, lb, le, fuse);
+    return factor_levels<float, float>(p, s, lb, le, fuse);
   };
+  // mixed: the x = 0 residual pass writes r = d in FP32 (the first solve's right-hand
+  // side), before the level-0 kernel reads it when that kernel runs the sweep
+  const bss::MixedSolveHooks hooks = mixed_hooks(p);
+  if (mixed && fuse && bss::mixed_first_pass(p->mx, hooks, s) != 0)
+    return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
   // levels 0 and 1 with their hand-offs on the caller's stream (ordered after the
   // caller's writes of A, B and d); the side stream starts when they are done.  Level 1
-  // still fills the device, so the side stream's sweeps (levels 0 and 1, then each
-  // later level once it is factored) only have idle compute units to use beside the
-  // later, latency-bound levels.
+  // still fills the device, so the side stream's sweeps (levels 0 -- unless fused -- and
+  // 1, then each later level once it is factored) only have idle compute units to use
+  // beside the later, latency-bound levels.
   const size_t split = 2;
   if (int e = factor(0, split)) return e;
   hipStream_t s2 = p->side_stream;
   HIP_TRY(hipEventRecord(p->ev_l0_factored, s));
   HIP_TRY(hipStreamWaitEvent(s2, p->ev_l0_factored, 0));
-  // side stream: the first solve's forward sweeps of the factored levels (mixed: after
-  // the x = 0 residual pass that writes r = d in FP32) beside the later levels'
-  // factorisation
-  const bss::MixedSolveHooks hooks = mixed_hooks(p);
-  if (mixed && bss::mixed_first_pass(p->mx, hooks, s2) != 0)
+  // side stream: the first solve's forward sweeps of the factored levels beside the
+  // later levels' factorisation
+  if (mixed && !fuse && bss::mixed_first_pass(p->mx, hooks, s2) != 0)
     return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
   hipError_t e = hipSuccess;
-  for (size_t l = 0; l < split && e == hipSuccess; ++l) {
+  for (size_t l = fuse ? 1 : 0; l < split && e == hipSuccess; ++l) {
     GLevel& G = p->gl[l];
     const bss::SolveLevelArgs a{G.dchains, G.dtab[0], p->nrhs};
     ProfScope ps(p, BSS_K_FORWARD, (int)l, s2);

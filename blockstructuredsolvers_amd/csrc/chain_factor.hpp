@@ -38,7 +38,39 @@ struct FactorLevelArgs {
   int* info;               // per system, atomicMin of the 1-based failing block index
   unsigned* pivmin;        // per system, atomicMin of the float bits of min_j U_jj^2 / A_jj
                            // over the level's blocks (null: not monitored)
+  int fwd = 0;             // 1: also run this level's forward sweep for one right-hand
+                           // side (chain_forward1_kernel's arithmetic, on the factors
+                           // while they are in LDS): d <- y in place, r_R and r_L out
+                           // (chain_factor_kernel only; see factor_fuses_forward)
 };
+
+// The forward sweep's dot product (XOps::dot of chain_solve.hpp) with the factor's
+// column in LDS: (M^T y)[row] over the 16-deep slices [s_lo, s_hi), x[k] = M[k][row],
+// lane (lr, lk) holding k = 16 s + 4 lk + q, two partial sums, then the sum over the
+// four lanes of the row; the same operations in the same order, so the same bits.
+template <typename T>
+__device__ inline T fwd_dot(const T* __restrict__ x, const T* __restrict__ y, int s_lo, int s_hi) {
+  const int lk = lane_id() >> 4;
+  T t0 = T(0), t1 = T(0);
+  for (int s = s_lo; s < s_hi; ++s) {
+    t0 = fma(x[16 * s + 4 * lk + 0], y[16 * s + 4 * lk + 0], t0);
+    t1 = fma(x[16 * s + 4 * lk + 1], y[16 * s + 4 * lk + 1], t1);
+    t0 = fma(x[16 * s + 4 * lk + 2], y[16 * s + 4 * lk + 2], t0);
+    t1 = fma(x[16 * s + 4 * lk + 3], y[16 * s + 4 * lk + 3], t1);
+  }
+  return row4_sum(t0 + t1);
+}
+// ... and its one-accumulator form (the sweep's C_last^T y_last)
+template <typename T, int TT>
+__device__ inline T fwd_dot1(const T* __restrict__ x, const T* __restrict__ y) {
+  const int lk = lane_id() >> 4;
+  T t = T(0);
+#pragma unroll
+  for (int s = 0; s < TT; ++s)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t = fma(x[16 * s + 4 * lk + q], y[16 * s + 4 * lk + q], t);
+  return row4_sum(t);
+}
 
 template <typename T>
 struct LdsPad {
@@ -394,7 +426,7 @@ constexpr bool factor_has_latency_variant() {
   return factor_min_waves_per_eu<T, NB, NW>() > 1;
 }
 
-template <typename Tin, typename T, int NB, int NW, bool LAT = false>
+template <typename Tin, typename T, int NB, int NW, bool LAT = false, bool FWD = false>
 __global__ void __launch_bounds__(64 * NW)
     __attribute__((amdgpu_waves_per_eu(LAT ? 1 : factor_min_waves_per_eu<T, NB, NW>())))
     chain_factor_kernel(FactorLevelArgs p) {
@@ -403,7 +435,10 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr int NT = 64 * NW;
   constexpr int NN = NB * NB;
   constexpr int NS = TT * (TT + 1) / 2;           // upper tiles of a symmetric block
-  constexpr int SQ = (NS + NW - 1) / NW;          // S accumulators per wave
+  // waves with the products that follow the trsm (S_L, Z, Schur, reduced-system blocks):
+  // all of them, or with the fused forward sweep all but its wave
+  constexpr int NWP = (FWD && NW > 1) ? NW - 1 : NW;
+  constexpr int SQ = (NS + NWP - 1) / NWP;        // S accumulators per wave
   static_assert(NW % TT == 0, "waves must be a multiple of the tile rows");
   // SLOTS (FP32 factors, n = 64, eight waves, where the 128-register cap leaves no room
   // to hold a prefetched block through the potrf): the next blocks' HBM loads are
@@ -416,14 +451,17 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr int NTB = SLOTS ? 64 * (NW - 1) : 64;  // slot threads (unused without SLOTS)
   constexpr int XG = NW / TT;                      // wave groups splitting the RHS columns
   constexpr int XQ = 2 * TT / XG;                  // trsm output tiles per wave
-  constexpr int ZQ = (TT * TT + NW - 1) / NW;     // panel tiles per wave
-  constexpr int PQ = (TT * TT + NS + NW - 1) / NW;
+  constexpr int ZQ = (TT * TT + NWP - 1) / NWP;   // panel tiles per wave
+  constexpr int PQ = (TT * TT + NS + NWP - 1) / NWP;
   typedef typename Mfma<T>::acc_t acc_t;
 
   __shared__ T Abuf[NB * LD];
   __shared__ T Ubuf[NB * LD];
   __shared__ T Rbuf[2 * NB * LD];  // columns [0,NB): B -> C ; [NB,2NB): Z -> W
   __shared__ int fail;
+  // fused forward sweep: r and y of the current block, C^T y of the previous block, the
+  // running sum W^T y
+  __shared__ T Fr[NB], Fy[NB], Ft[NB], Fl[NB], Fd[NB];  // Fd: d of the current block
 
   const ChainRec cd = p.chains[blockIdx.x];
   const int start = cd.start, len = cd.len, sepL = cd.sepL, sepR = cd.sepR;
@@ -455,6 +493,22 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr bool PF = !SLOTS && (LAT || factor_prefetch<T, NB, NW>());
   if (PF && (len > 1 || hasR)) sb.load(B + (long long)start * NN);
   sa.template store<T, LD, false>(Abuf);
+  // FWD, the fused forward sweep (one right-hand side, the variant launched for
+  // p.fwd): the last wave, which has the fewest product tasks, runs
+  // chain_forward1_kernel's steps for every row tile of each block once its factors are
+  // in LDS; lane (lr, lk) of row tile t holds what lane (lr, lk) of that kernel's wave
+  // t holds
+  const bool fwd = FWD && w == NW - 1;
+  const auto dg = gptr((T*)S.d);
+  const int flr = lane_id() & 15, flk = lane_id() >> 4;
+  constexpr int FDQ = (NB + 63) / 64;  // d elements per lane of the sweep's wave
+  if (fwd) {
+#pragma unroll
+    for (int q = 0; q < FDQ; ++q) {
+      const int row = lane_id() + 64 * q;
+      if (row < NB) Fd[row] = dg[(long long)start * NB + row];
+    }
+  }
   __syncthreads();
 
   for (int i = 0; i < len; ++i) {
@@ -545,6 +599,62 @@ __global__ void __launch_bounds__(64 * NW)
       }
     }
     __syncthreads();
+    if (fwd) {
+      const int blk1 = start + i;
+      // lane offsets recomputed per block (held across the chain they would spill)
+      const int flr = opaque_v(lane_id() & 15), flk = lane_id() >> 4;
+      // d of the next block, one element per lane, in flight through this step's
+      // products (registers held within this step only)
+      T nd[FDQ];
+#pragma unroll
+      for (int q = 0; q < FDQ; ++q) {
+        const int row = lane_id() + 64 * q;
+        nd[q] = (!last && row < NB) ? dg[(long long)(blk1 + 1) * NB + row] : T(0);
+      }
+      // r = d_blk - C_{blk-1}^T y_{blk-1}
+#pragma unroll
+      for (int t = 0; t < TT; ++t) {
+        const int row = 16 * t + flr;
+        T v = Fd[row];
+        if (i > 0) v -= Ft[row];
+        if (flk == 0) Fr[row] = v;
+      }
+      // y = Uinv^T r (tiles t' <= t of column tile t), in place of d; the row-tile loops
+      // stay rolled, which keeps this step's registers few (the accumulators of the
+      // other waves' products share the budget)
+#pragma unroll 1
+      for (int t = 0; t < TT; ++t) {
+        const int row = 16 * t + flr;
+        const T y = fwd_dot(Ubuf + row * LD, Fr, 0, t + 1);
+        if (flk == 0) {
+          dg[(long long)blk1 * NB + row] = y;
+          Fy[row] = y;
+        }
+      }
+      // r_L += W^T y;  C^T y for the next block, or r_R after the last one
+#pragma unroll 1
+      for (int t = 0; t < TT; ++t) {
+        const int row = 16 * t + flr;
+        if (hasL) {
+          const T v = fwd_dot(Rbuf + (NB + row) * LD, Fy, 0, TT);
+          if (flk == 0) Fl[row] = (i > 0 ? Fl[row] : T(0)) + v;
+        }
+        if (!last) {
+          const T v = fwd_dot(Rbuf + row * LD, Fy, 0, TT);
+          if (flk == 0) Ft[row] = v;
+        } else if (hasR) {
+          const T v = fwd_dot1<T, TT>(Rbuf + row * LD, Fy);
+          if (flk == 0) gptr((T*)S.rR)[(long long)sepR * NB + row] = v;
+        }
+      }
+      if (!last) {
+#pragma unroll
+        for (int q = 0; q < FDQ; ++q) {
+          const int row = lane_id() + 64 * q;
+          if (row < NB) Fd[row] = nd[q];
+        }
+      }
+    }
     // factors to HBM as flat 16-byte stores: C (B slot), W (panel slot), Uinv upper tiles
     {
       typedef T vt __attribute__((ext_vector_type(16 / sizeof(T))));
@@ -565,10 +675,10 @@ __global__ void __launch_bounds__(64 * NW)
     }
 
     // S_L += W^T W (upper tiles, persistent per wave)
-    if (hasL) {
+    if (hasL && (NWP == NW || wb < NWP)) {
 #pragma unroll
       for (int q = 0; q < SQ; ++q) {
-        int s = wb + NW * q;
+        int s = wb + NWP * q;
         if (s < NS) {
           int a = 0;
           while (s >= TT - a) {
@@ -586,7 +696,7 @@ __global__ void __launch_bounds__(64 * NW)
       acc_t Z[ZQ];
 #pragma unroll
       for (int q = 0; q < PQ; ++q) {
-        const int task = wb + NW * q;
+        const int task = (NWP == NW || wb < NWP) ? wb + NWP * q : TT * TT + NS;
         if (task < TT * TT) {
           if (hasL && q < ZQ) {
             const int a = task % TT, b = task / TT;
@@ -610,7 +720,7 @@ __global__ void __launch_bounds__(64 * NW)
       if (hasL) {
 #pragma unroll
         for (int q = 0; q < ZQ; ++q) {
-          const int task = wb + NW * q;
+          const int task = (NWP == NW || wb < NWP) ? wb + NWP * q : TT * TT;
           if (task < TT * TT) {
             const int a = task % TT, b = task / TT;
             tile_store<T>(Rbuf + tile_off(a, TT + b, LD), LD, Z[q]);
@@ -621,7 +731,7 @@ __global__ void __launch_bounds__(64 * NW)
       // reduced-system contributions of the last block
       if (hasR) {
         T* sr = (T*)S.SR + (long long)sepR * NN;
-        for (int task = w; task < NS; task += NW) {
+        for (int task = w; (NWP == NW || w < NWP) && task < NS; task += NWP) {
           int s = task, a = 0;
           while (s >= TT - a) {
             s -= TT - a;
@@ -634,7 +744,7 @@ __global__ void __launch_bounds__(64 * NW)
         }
         if (hasL) {
           T* bn = (T*)S.Bn + (long long)sepL * NN;
-          for (int task = w; task < TT * TT; task += NW) {
+          for (int task = w; (NWP == NW || w < NWP) && task < TT * TT; task += NWP) {
             const int a = task % TT, b = task / TT;
             acc_t acc = Mfma<T>::zero();
             tile_mma<T, true, false, true, NB>(acc, Rbuf + (NB + 16 * a) * LD, LD,
@@ -649,8 +759,8 @@ __global__ void __launch_bounds__(64 * NW)
     T* sl = (T*)S.SL + (long long)sepL * NN;
 #pragma unroll
     for (int q = 0; q < SQ; ++q) {
-      int s = w + NW * q;
-      if (s < NS) {
+      int s = w + NWP * q;
+      if ((NWP == NW || w < NWP) && s < NS) {
         int a = 0;
         while (s >= TT - a) {
           s -= TT - a;
@@ -660,6 +770,10 @@ __global__ void __launch_bounds__(64 * NW)
         tile_store<T>(sl + tile_off(a, b, NB), NB, Sacc[q]);
       }
     }
+  }
+  if (fwd && hasL && flk == 0) {
+#pragma unroll
+    for (int t = 0; t < TT; ++t) gptr((T*)S.rL)[(long long)sepL * NB + 16 * t + flr] = Fl[16 * t + flr];
   }
   // the chain's first failed block (original indices increase along a chain, so the
   // first local failure is the smallest original index) and its smallest pivot ratio

@@ -23,21 +23,34 @@ static int device_cus() {
   return cached[dev];
 }
 
-template <typename Tin, typename T, int NB, int NW>
-static hipError_t factor_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
+template <typename Tin, typename T, int NB, int NW, bool FWD>
+static hipError_t factor_nb_f(const FactorLevelArgs& a, int nchains, hipStream_t s) {
   if constexpr (factor_has_latency_variant<T, NB, NW>()) {
     if (nchains <= device_cus()) {  // one chain per CU at most: latency variant
-      hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW, true>), dim3(nchains),
+      hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW, true, FWD>), dim3(nchains),
                          dim3(64 * NW), 0, s, a);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
+  hipLaunchKernelGGL((chain_factor_kernel<Tin, T, NB, NW, false, FWD>), dim3(nchains),
+                     dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename Tin, typename T, int NB, int NW>
+static hipError_t factor_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
+  if (a.fwd) {  // fused forward sweep: FP32 factors only (factor_fuses_forward)
+    if constexpr (sizeof(T) == 4)
+      return factor_nb_f<Tin, T, NB, NW, true>(a, nchains, s);
+    else
+      return hipErrorInvalidValue;
+  }
+  return factor_nb_f<Tin, T, NB, NW, false>(a, nchains, s);
+}
+
+template <typename Tin, typename T, int NB, int NW>
 static hipError_t factor_big_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
+  if (a.fwd) return hipErrorInvalidValue;  // no fused forward sweep (factor_fuses_forward)
   hipLaunchKernelGGL((chain_factor_big_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
