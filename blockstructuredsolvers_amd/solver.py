@@ -568,9 +568,11 @@ def solve(data: BlockTriDiagData_batched) -> int:
 
 def factorize_solve(data: BlockTriDiagData_batched) -> int:
     """``factorize!(data); solve!(data)`` (src/gpu_batched.jl:184-259) in one call, for
-    a d that is final before the factorisation: the first solve's forward sweeps
-    overlap the separator levels' factorisation on a second stream.  Same results,
-    state and status as ``factorize`` then ``solve``; asynchronous."""
+    a d that is final before the factorisation: with FP32 factors and one right-hand
+    side the level-0 factor kernel also runs the first solve's level-0 forward sweep;
+    the other forward sweeps overlap the separator levels' factorisation on a second
+    stream.  Same results (bitwise), state and status as ``factorize`` then ``solve``;
+    asynchronous."""
     with torch.cuda.device(data.device):
         rc = L.call("bss_factorize_solve", data._plan, _stream_handle(data.device))
     data.last_status = rc
