@@ -322,7 +322,13 @@ __device__ __attribute__((always_inline)) inline void potrf_inv(T* Abuf, T* Ubuf
           tile_store<T>(Abuf + tile_off(i, i, LD), LD, acc);
           diag(i);
         } else {
-          eliminate(kk, w, NW - 1);
+          // with eight waves, wave 4 shares wave 0's SIMD, where an MFMA stalls the
+          // diagonal steps: the other six waves take the elimination tiles
+          if constexpr (NW == 8) {
+            if (w != 4) eliminate(kk, w - (w > 4), NW - 2);
+          } else {
+            eliminate(kk, w, NW - 1);
+          }
           if (SL) slot(kk + 1);
         }
         __syncthreads();
