@@ -787,8 +787,9 @@ int create(int nsys, const int* Ns, int n, int nrhs, int dtype, int mode, int sc
 }
 
 // Refinement-loop hooks over this plan.  The first solve32 after bss_factorize_solve
-// (vector set 0, fwd_done set) skips the forward sweeps that already ran on the side
-// stream, and runs without a graph.  That first solve32 always runs on the FP32
+// (vector set 0, fwd_done set) skips the forward sweeps that already ran (level 0's
+// inside its factor kernel when fused, the others on the side stream), and runs
+// without a graph.  That first solve32 always runs on the FP32
 // separators: the pivot check (prepare) follows it, and an FP64 separator
 // re-factorisation there makes the loop redo it from d on the FP64 separators.
 bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
