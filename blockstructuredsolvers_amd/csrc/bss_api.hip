@@ -23,6 +23,7 @@
 #include "launch.hpp"
 #include "plan_tables.hpp"
 #include "mixed.hpp"
+#include "pinned_pool.hpp"
 #include "schedule.hpp"
 
 namespace {
@@ -153,6 +154,8 @@ struct bss_plan {
   };
   std::vector<TabOff> toff;
   size_t info_off = kNone;
+  // host-written set-up ranges of the workspace (see layout)
+  size_t static_begin = 0, static_end = 0, bound_begin = 0, bound_end = 0;
   // separator re-factorisation (mixed mode)
   size_t widen_off = kNone, narrow_off = kNone;
   unsigned* dpiv = nullptr;
@@ -185,7 +188,7 @@ struct bss_plan {
   ~bss_plan() {
     drop_graphs();
     if (piv_event) (void)hipEventDestroy(piv_event);
-    if (host_piv) (void)hipHostFree(host_piv);
+    if (host_piv) bss::pinned_give(host_piv);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (ev_l0_factored) (void)hipEventDestroy(ev_l0_factored);
     if (ev_l0_solved) (void)hipEventDestroy(ev_l0_solved);
@@ -296,8 +299,6 @@ int layout(bss_plan* p) {
         V.orR = bump.take(V.P * nd * fs);
         V.orL = bump.take(V.P * nd * fs);
       }
-      V.oOrig = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
-      if (l > 0) V.oPrev = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
       if (mixed && l > 0) {
         // the FP32 factor keeps its C apart from B, so the reduced couplings survive
         // for an FP64 re-factorisation
@@ -332,24 +333,46 @@ int layout(bss_plan* p) {
       for (int k = 0; k < V.P; ++k) G.seps.push_back({slot, k, V.shape.seppos[k], 0});
     }
   }
+  // Everything the host writes sits in two contiguous ranges, each uploaded in one copy:
+  // the static range (index maps, chain and separator records, the tables of levels
+  // >= 2, conversion records), written once when the workspace is first set up, and the
+  // bound range (the tables of levels 0 and 1, which name the caller's buffers),
+  // rewritten at every bind.
+  p->static_begin = bump.off;
+  for (int s = 0; s < p->nsys; ++s) {
+    SysPlan& S = p->sp[s];
+    for (size_t l = 0; l < S.lv.size(); ++l) {
+      SysLevel& V = S.lv[l];
+      const size_t NL = (size_t)V.shape.N;
+      V.oOrig = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
+      if (l > 0) V.oPrev = bump.take(sizeof(int) * std::max<size_t>(NL, 1));
+    }
+  }
   p->toff.assign(L, bss_plan::TabOff());
-  for (size_t l = 0; l < L; ++l) {
-    GLevel& G = p->gl[l];
-    p->toff[l].chains = bump.take(sizeof(bss::ChainRec) * std::max<size_t>(G.chains.size(), 1));
-    p->toff[l].seps = bump.take(sizeof(bss::SepRec) * std::max<size_t>(G.seps.size(), 1));
+  auto take_tables = [&](size_t l) {
+    const GLevel& G = p->gl[l];
     for (int k = 0; k < (l <= 1 ? 2 : 1); ++k) {
       p->toff[l].tab[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
       if (mixed) p->toff[l].tab64[k] = bump.take(sizeof(bss::LevelSys) * G.slot_sys.size());
     }
+  };
+  for (size_t l = 0; l < L; ++l) {
+    GLevel& G = p->gl[l];
+    p->toff[l].chains = bump.take(sizeof(bss::ChainRec) * std::max<size_t>(G.chains.size(), 1));
+    p->toff[l].seps = bump.take(sizeof(bss::SepRec) * std::max<size_t>(G.seps.size(), 1));
+    if (l >= 2) take_tables(l);
   }
-  // status words: [0, nsys) info of level 0 and FP64 levels, [nsys, 2 nsys) info of the
-  // FP32 separator levels; mixed mode adds [2 nsys, 3 nsys) pivot monitor, so that the
-  // host reads separator info and pivots in one copy
-  p->info_off = bump.take(sizeof(int) * (mixed ? 3 : 2) * p->nsys);
   if (mixed) {
     p->widen_off = bump.take(sizeof(bss::ConvRec<float, double>) * p->nsys);
     p->narrow_off = bump.take(sizeof(bss::ConvRec<double, float>) * p->nsys);
   }
+  p->static_end = p->bound_begin = bump.off;
+  for (size_t l = 0; l < std::min<size_t>(L, 2); ++l) take_tables(l);
+  p->bound_end = bump.off;
+  // status words: [0, nsys) info of level 0 and FP64 levels, [nsys, 2 nsys) info of the
+  // FP32 separator levels; mixed mode adds [2 nsys, 3 nsys) pivot monitor, so that the
+  // host reads separator info and pivots in one copy
+  p->info_off = bump.take(sizeof(int) * (mixed ? 3 : 2) * p->nsys);
   if (mixed) {
     std::vector<int> nu(p->nsys);
     for (int s = 0; s < p->nsys; ++s) nu[s] = p->sp[s].N_use;
@@ -441,18 +464,49 @@ __global__ void zero_kernel(uint4* __restrict__ p, size_t n16) {
     p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-int upload_tables(bss_plan* p, size_t l) {
-  GLevel& G = p->gl[l];
+// Host image of one set-up range of the workspace: every piece is written at its
+// workspace offset, and the range then goes to the device in one copy.
+struct SetupImage {
+  size_t base;
+  std::vector<char> bytes;
+  bool ok = true;
+  SetupImage(size_t begin, size_t end) : base(begin), bytes(end - begin, 0) {}
+  void put(size_t off, const void* src, size_t n) {
+    if (!n) return;
+    if (off < base || off + n > base + bytes.size()) {
+      ok = false;
+      return;
+    }
+    std::memcpy(bytes.data() + (off - base), src, n);
+  }
+  void put_at(const bss_plan* p, const void* dev, const void* src, size_t n) {
+    put((size_t)((const char*)dev - p->ws), src, n);
+  }
+  int upload(const bss_plan* p) const {
+    if (!ok) return fail(BSS_E_STATE, "set-up data outside its workspace range");
+    HIP_TRY(h2d(p, p->ws + base, bytes.data(), bytes.size()));
+    return BSS_OK;
+  }
+};
+
+void fill_tables(const bss_plan* p, size_t l, SetupImage& img) {
+  const GLevel& G = p->gl[l];
   for (int k = 0; k < (l <= 1 ? 2 : 1); ++k) {
     for (int wide = 0; wide < (p->mode == BSS_MODE_MIXED ? 2 : 1); ++wide) {
       if (wide && l == 1 && k == 1) continue;  // FP64 tables vary at level 0 only
       std::vector<bss::LevelSys> tab;
       for (int s : G.slot_sys) tab.push_back(level_sys(p, s, l, k, wide != 0));
-      HIP_TRY(h2d(p, wide ? G.dtab64[k] : G.dtab[k], tab.data(),
-                  tab.size() * sizeof(bss::LevelSys)));
+      img.put_at(p, wide ? G.dtab64[k] : G.dtab[k], tab.data(),
+                 tab.size() * sizeof(bss::LevelSys));
     }
   }
-  return BSS_OK;
+}
+
+// The tables of levels 0 and 1, which name the caller's buffers, in one copy.
+int upload_bound_tables(bss_plan* p) {
+  SetupImage img(p->bound_begin, p->bound_end);
+  for (size_t l = 0; l < std::min<size_t>(2, p->gl.size()); ++l) fill_tables(p, l, img);
+  return img.upload(p);
 }
 
 // Layout + workspace allocation + static tables (device work).
@@ -497,8 +551,13 @@ int build(bss_plan* p) {
       if (p->mode == BSS_MODE_MIXED)
         G.dtab64[k] = (bss::LevelSys*)(p->ws + p->toff[l].tab64[l == 0 ? k : 0]);
     }
-    HIP_TRY(h2d(p, G.dchains, G.chains.data(), G.chains.size() * sizeof(bss::ChainRec)));
-    HIP_TRY(h2d(p, G.dseps, G.seps.data(), G.seps.size() * sizeof(bss::SepRec)));
+  }
+  // the static set-up range, assembled on the host and uploaded in one copy
+  SetupImage img(p->static_begin, p->static_end);
+  for (size_t l = 0; l < p->gl.size(); ++l) {
+    const GLevel& G = p->gl[l];
+    img.put_at(p, G.dchains, G.chains.data(), G.chains.size() * sizeof(bss::ChainRec));
+    img.put_at(p, G.dseps, G.seps.data(), G.seps.size() * sizeof(bss::SepRec));
   }
   // original (level-0) block index of every block, per system
   for (int s = 0; s < p->nsys; ++s) {
@@ -507,12 +566,12 @@ int build(bss_plan* p) {
     for (size_t i = 0; i < orig.size(); ++i) orig[i] = (int)i;
     for (size_t l = 0; l < S.lv.size(); ++l) {
       SysLevel& V = S.lv[l];
-      HIP_TRY(h2d(p, p->ws + V.oOrig, orig.data(), orig.size() * sizeof(int)));
+      img.put(V.oOrig, orig.data(), orig.size() * sizeof(int));
       std::vector<int> next(V.P);
       for (int k = 0; k < V.P; ++k) next[k] = orig[V.shape.seppos[k]];
       orig.swap(next);
       if (l + 1 < S.lv.size() && V.P > 0)  // next level's block q sits at seppos[q] here
-        HIP_TRY(h2d(p, p->ws + S.lv[l + 1].oPrev, V.shape.seppos.data(), V.P * sizeof(int)));
+        img.put(S.lv[l + 1].oPrev, V.shape.seppos.data(), V.P * sizeof(int));
     }
   }
   if (p->mode == BSS_MODE_MIXED) {
@@ -533,16 +592,15 @@ int build(bss_plan* p) {
                     (long long)V.shape.N * nd});
     }
     p->nconv = (int)wr.size();
-    HIP_TRY(hipHostMalloc((void**)&p->host_piv, sizeof(unsigned) * 3 * p->nsys));
+    img.put_at(p, p->dwiden, wr.data(), wr.size() * sizeof(wr[0]));
+    img.put_at(p, p->dnarrow, nr.data(), nr.size() * sizeof(nr[0]));
+    p->host_piv = (unsigned*)bss::pinned_take(sizeof(unsigned) * 3 * p->nsys);
+    if (!p->host_piv) return fail(BSS_E_NOMEM, "pinned host allocation failed");
     HIP_TRY(hipEventCreateWithFlags(&p->piv_event, hipEventDisableTiming));
-    if (!wr.empty()) {
-      HIP_TRY(hipMemcpy(p->dwiden, wr.data(), wr.size() * sizeof(wr[0]), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(p->dnarrow, nr.data(), nr.size() * sizeof(nr[0]), hipMemcpyHostToDevice));
-    }
   }
+  for (size_t l = 2; l < p->gl.size(); ++l) fill_tables(p, l, img);
+  if (int e = img.upload(p)) return e;
   p->built = true;
-  for (size_t l = 1; l < p->gl.size(); ++l)
-    if (int e2 = upload_tables(p, l)) return e2;
   return BSS_OK;
 }
 
@@ -1168,8 +1226,7 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
   if (p->mode == BSS_MODE_MIXED && p->mx.bind((const double*)A, (const double*)B, (const double*)d) != 0)
     return fail(BSS_E_HIP, "mixed bind: %s", bss::mixed_last_error());
   // levels 0 and 1 name the caller's buffers (level 1 reads level 0's vectors)
-  for (size_t l = 0; l < std::min<size_t>(2, p->gl.size()); ++l)
-    if (int e = upload_tables(p, l)) return e;
+  if (int e = upload_bound_tables(p)) return e;
   HIP_TRY(hipStreamSynchronize(nullptr));
   p->factored = false;
   return BSS_OK;
@@ -1338,8 +1395,8 @@ int bss_factorize_solve_noalloc(int N, int n, int nrhs, int dtype, const void* A
   P.B = (void*)B;
   P.d = d;
   P.last_stream = s;
-  if (int e = build(&P)) return e;  // clears scratch, uploads tables of levels >= 1
-  if (int e = upload_tables(&P, 0)) return e;
+  if (int e = build(&P)) return e;  // clears scratch, uploads the static tables
+  if (int e = upload_bound_tables(&P)) return e;
   int rc = dtype == BSS_F64 ? factor_levels<double, double>(&P, s)
                             : factor_levels<float, float>(&P, s);
   if (rc == BSS_OK)

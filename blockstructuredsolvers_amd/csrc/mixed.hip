@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "mixed.hpp"
+#include "pinned_pool.hpp"
 
 namespace bss {
 namespace {
@@ -538,8 +539,7 @@ int MixedState::attach(char* base, const std::vector<int>& Ns_, int n_, int nrhs
     return -1;
   }
   if (!host_berr &&
-      hipHostMalloc((void**)&host_berr, (size_t)nsys * 16 * sizeof(double)) != hipSuccess) {
-    host_berr = nullptr;
+      !(host_berr = (double*)pinned_take((size_t)nsys * 16 * sizeof(double)))) {
     g_mixed_err = "pinned host allocation failed";
     return -1;
   }
@@ -573,7 +573,7 @@ int MixedState::bind(const double* A, const double* B, const double* d) {
 }
 
 void MixedState::release() {
-  if (host_berr) (void)hipHostFree(host_berr);
+  pinned_give(host_berr);
   host_berr = nullptr;
   if (berr_ready) (void)hipEventDestroy(berr_ready);
   berr_ready = nullptr;
