@@ -415,16 +415,20 @@ def batch_line(args, torch, oracle):
 def serve_lines(args):
     """Served stream (Poisson arrivals of smoother systems, n in {16, 32, 64}, N ~
     U[128, 4096], 1-8 right-hand sides per factorisation, blocks built on the device):
-    throughput and p50/p99 latency at a light and a saturating rate, plain and mixed."""
+    throughput and p50/p99 latency at a light and a saturating rate, plain and mixed.
+    The saturating rate's line also carries the group-plan rate of the same requests
+    (all present at once, one group plan per block size and column class)."""
     from blockstructuredsolvers_amd import serve_bench
     out = []
     for mode in ("plain", "mixed"):
         for rate in args.serve_rates:
-            r = serve_bench.run(rate, args.serve_count, mode, workers=8, seed=0, N_max=4096)
+            top = rate == max(args.serve_rates)
+            r = serve_bench.run(rate, args.serve_count, mode, workers=8, seed=0, N_max=4096,
+                                group_plan=top)
             keep = {k: r[k] for k in ("mode", "rate_per_s", "requests", "systems_per_s",
                                       "latency_ms_p50", "latency_ms_p99", "solve_ms_p50",
                                       "solve_ms_p99", "queue_ms_p50", "failed", "groups",
-                                      "max_group")}
+                                      "max_group", "group_plan_systems_per_s") if k in r}
             out.append(keep)
             log(f"bench: served {keep}")
     return out

@@ -20,7 +20,10 @@ from . import serving
 
 
 def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
-        device_blocks: bool = True, max_group: int = 64) -> dict:
+        device_blocks: bool = True, max_group: int = 64, group_plan: bool = False) -> dict:
+    """Serve `count` requests arriving at `rate`/s; with ``group_plan`` also the rate of
+    the same requests run at once as one group plan per block size and column class
+    (``serving.group_plan_rate``), the saturation bound of the loop."""
     reqs = serving.smoother_requests(count, rate, N_range=(128, N_max), seed=seed,
                                      device_blocks=device_blocks)
     srv = serving.Server(mode=mode, workers=workers, max_group=max_group)
@@ -36,6 +39,8 @@ def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
                groups=srv.stats["groups"], max_group=srv.stats["max_group"],
                max_inflight_gb=srv.stats["max_inflight_bytes"] / 1e9,
                rhs_total=sum(len(r.rhs) for r in reqs), failed=sum(r.info != 0 for r in res))
+    if group_plan:
+        out["group_plan_systems_per_s"] = serving.group_plan_rate(srv, reqs)
     return out
 
 
@@ -49,6 +54,8 @@ def main(argv=None) -> int:
     ap.add_argument("--N-max", type=int, default=4096)
     ap.add_argument("--max-group", type=int, default=64,
                     help="requests coalesced into one group plan at most (1: none)")
+    ap.add_argument("--group-plan", action="store_true",
+                    help="also time the same requests as one group plan per class")
     ap.add_argument("--upload-blocks", action="store_true",
                     help="send dense A, B with each request instead of the model")
     a = ap.parse_args(argv)
@@ -57,7 +64,7 @@ def main(argv=None) -> int:
         return 2
     for r in [float(x) for x in a.rates.split(",")]:
         print(json.dumps(run(r, a.count, a.mode, a.workers, a.seed, a.N_max,
-                             not a.upload_blocks, a.max_group)), flush=True)
+                             not a.upload_blocks, a.max_group, a.group_plan)), flush=True)
     return 0
 
 

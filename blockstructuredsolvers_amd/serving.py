@@ -14,25 +14,29 @@ around the same plans:
   HBM budget beside the requests in flight.  There is no batch boundary: a request
   that arrives while others run starts as soon as a slot and memory free up.
 * Requests are coalesced: when a worker is free, the FIFO head is admitted together
-  with every other waiting request of the same block size n and right-hand-side width
-  (up to ``max_group`` requests and the HBM budget), and the group runs as one group
-  plan (``bss_plan_create_group``), so its systems share every launch.
+  with every other waiting request of the same block size n and column class (up to
+  ``max_group`` requests and the HBM budget), and the group runs as one group plan
+  (``bss_plan_create_group``), so its systems share every launch.  A request's columns
+  are its right-hand sides side by side (``width`` = count x nrhs).  Requests of one
+  column go to one-column plans (the single-column sweeps); wider requests go to plans
+  as wide as the widest request of the group, at most 16 columns, and a narrower
+  request fills its spare columns with copies of its first one.
 * Each worker thread owns one HIP stream and runs its group end to end: blocks
   uploaded (or built on the device from the model), one ``factorize`` of all systems,
-  then solve rounds: round k solves the k-th right-hand side of every request that has
-  one (one ``solve`` over the group) and downloads those solutions.  The factors stay
-  resident between the rounds.  Work of different workers overlaps on the device
-  (ctypes releases the GIL during library calls).  Plans take their workspace from
-  torch's caching allocator, so admission and release make no device allocation or
-  free.
+  then one ``solve`` of every request's columns at once (rounds of 16 columns for
+  wider requests), and the downloads.  The factors stay resident between rounds.
+  Work of different workers overlaps on the device (ctypes releases the GIL during
+  library calls).  Plans take their workspace from torch's caching allocator, so
+  admission and release make no device allocation or free.
 * Latency of a request = host time its last solution is available - arrival time; a
-  solve's latency is its round's device time.
+  solve's latency is the device time of the solve that produced it.
 
 A system's arithmetic in a group plan is the arithmetic it gets alone (its own level
-schedule, separate launch slots, no cross-system reductions), so a served answer equals
-the answer of the same request solved alone (bitwise for plain plans; within the
-refinement tolerance in mixed mode, where a group refines until all its systems meet
-the target).
+schedule, separate launch slots, no cross-system reductions), and the multi-column
+sweeps compute every column on its own, whatever the width, so a served answer equals
+the answer of the same request solved alone (``solve_alone``; bitwise for plain plans,
+within the refinement tolerance in mixed mode, where a group refines until all its
+systems meet the target).
 """
 from __future__ import annotations
 
@@ -76,6 +80,11 @@ class Request:
     def nrhs(self) -> int:
         return self.rhs[0].shape[1]
 
+    @property
+    def width(self) -> int:
+        """Columns to solve: the right-hand sides side by side."""
+        return len(self.rhs) * self.nrhs
+
     def dense(self):
         """(A, B) as host arrays (from the model when the request carries one)."""
         if self.A is not None:
@@ -90,15 +99,32 @@ class Request:
 class Result:
     rid: int
     x: List[np.ndarray]                 # one (n, nrhs, N) solution per right-hand side
+                                        # (views of a pinned host buffer)
     arrival: float
     admitted: float
     finished: float
-    solve_ms: List[float] = field(default_factory=list)  # per right-hand side, device
+    solve_ms: List[float] = field(default_factory=list)  # per right-hand side: device
+                                                         # time of the solve that made it
     info: int = 0
 
     @property
     def latency_ms(self) -> float:
         return (self.finished - self.arrival) * 1e3
+
+
+MAX_COLS = 16  # columns one solve takes (the plans' nrhs limit)
+
+
+def solve_width(width: int) -> int:
+    """Plan width for a request of `width` columns: 1 for a single column (the
+    single-column sweeps), otherwise the columns up to 16 per solve (the multi-column
+    sweeps)."""
+    return 1 if width == 1 else min(MAX_COLS, width)
+
+
+def group_width(group: List["Request"]) -> int:
+    """Plan width of a coalesced group (all one-column requests, or all wider)."""
+    return solve_width(max(r.width for r in group))
 
 
 def _storage_order(blocks: np.ndarray) -> torch.Tensor:
@@ -134,19 +160,34 @@ def _storage_view(t: torch.Tensor) -> torch.Tensor:
     return t.permute(2, 1, 0)
 
 
-def _build_smoother_into(data, A: torch.Tensor, B: torch.Tensor, req: Request) -> None:
+def _model_host(model) -> torch.Tensor:
+    """F, H, Q^-1, R^-1 of a state-space model, each column-major, packed one after the
+    other into one pinned FP64 buffer (one host-to-device copy per request)."""
+    parts = [np.asarray(M, dtype=np.float64).ravel(order="F")
+             for M in (model.F, model.H, model.Qinv, model.Rinv)]
+    return torch.from_numpy(np.concatenate(parts)).pin_memory()
+
+
+def _build_smoother_into(data, A: torch.Tensor, B: torch.Tensor, req: Request,
+                         host: Optional[torch.Tensor] = None) -> None:
     """bss_build_smoother into one system's A, B views of a group plan, on the current
-    stream; the model's device copies stay referenced by `data` until it is closed."""
-    n = req.n
+    stream.  `host` is the model packed by ``_model_host`` (packed here when absent);
+    its device copy stays referenced by `data` until the plan is closed."""
+    n, m = req.n, req.model.H.shape[0]
     dev = data.device
-    F, H, Qi, Ri = [torch.from_numpy(np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)).to(dev)
-                    for M in (req.model.F, req.model.H, req.model.Qinv, req.model.Rinv)]
+    if host is None:
+        host = _model_host(req.model)
+    buf = host.to(dev, non_blocking=True)
+    base, es = buf.data_ptr(), buf.element_size()
+    oF, oH = 0, n * n
+    oQ = oH + m * n
+    oR = oQ + n * n
     code = L.BSS_F64 if data.dtype == torch.float64 else L.BSS_F32
-    L.call("bss_build_smoother", req.N, n, req.model.H.shape[0], F.data_ptr(), H.data_ptr(),
-           Qi.data_ptr(), Ri.data_ptr(), A.data_ptr(), B.data_ptr() if req.N > 1 else None,
-           code, torch.cuda.current_stream(dev).cuda_stream)
+    L.call("bss_build_smoother", req.N, n, m, base + oF * es, base + oH * es, base + oQ * es,
+           base + oR * es, A.data_ptr(), B.data_ptr() if req.N > 1 else None, code,
+           torch.cuda.current_stream(dev).cuda_stream)
     keep = getattr(data, "_model_tensors", [])
-    keep.append((F, H, Qi, Ri))
+    keep.append(buf)
     data._model_tensors = keep
 
 
@@ -170,17 +211,20 @@ def group_footprint(Ns: List[int], n: int, nrhs: int, dtype=torch.float64,
 
 
 class _Staged:
-    """A request's host data in pinned memory, prepared before it is served (no A, B
-    when the device builds the blocks from the request's model)."""
+    """A request's host data in pinned memory, prepared before it is served: its dense
+    blocks, or its packed model when the device builds the blocks; its right-hand sides;
+    and one download buffer per right-hand side."""
 
     def __init__(self, req: Request, dtype):
         npt = np.float64 if dtype == torch.float64 else np.float32
         self.A = self.B = None
+        self.model = _model_host(req.model) if req.model is not None else None
         if req.model is None:
             self.A = _storage_order(req.A.astype(npt))
             self.B = _storage_order(req.B.astype(npt))
-        self.d = [_storage_order(d.astype(npt)) for d in req.rhs]
-        self.out = [torch.empty_like(d).pin_memory() for d in self.d]
+        # all columns side by side: (n, width, N) blocks in (N, width, n) storage order
+        self.d = _storage_order(np.concatenate([d.astype(npt) for d in req.rhs], axis=1))
+        self.out = torch.empty(self.d.shape, dtype=self.d.dtype, pin_memory=True)
 
 
 class Server:
@@ -210,61 +254,77 @@ class Server:
                       "max_group": 0}
 
     # -- admission ---------------------------------------------------------------------
-    def footprint(self, N: int, n: int, nrhs: int) -> int:
-        return request_footprint(N, n, nrhs, self.dtype, self.mode)
+    def footprint(self, r: Request) -> int:
+        """Device bytes request `r` holds when it runs alone."""
+        return request_footprint(r.N, r.n, solve_width(r.width), self.dtype, self.mode)
 
     def _group_fp(self, group: List[Request]) -> int:
         if len(group) == 1:
-            r = group[0]
-            return self.footprint(r.N, r.n, r.nrhs)
-        return group_footprint([r.N for r in group], group[0].n, group[0].nrhs, self.dtype,
-                               self.mode)
+            return self.footprint(group[0])
+        return group_footprint([r.N for r in group], group[0].n, group_width(group),
+                               self.dtype, self.mode)
+
+    @staticmethod
+    def _same_class(a: Request, b: Request) -> bool:
+        return a.n == b.n and (a.width > 1) == (b.width > 1)
 
     # -- one coalesced group on one worker stream -----------------------------------
     def _run(self, reqs: List[Request], staged: List[_Staged], stream: torch.cuda.Stream,
              t0: float, admitted: float) -> List[Result]:
-        n, nrhs = reqs[0].n, reqs[0].nrhs
+        n, P = reqs[0].n, group_width(reqs)
         with torch.cuda.stream(stream):
             data = initialize_batched_group([r.N for r in reqs], n, self.dtype, self.device,
-                                            nrhs=nrhs, mode=self.mode, torch_workspace=True,
+                                            nrhs=P, mode=self.mode, torch_workspace=True,
                                             **self.plan_options)
             try:
                 for k, (r, st) in enumerate(zip(reqs, staged)):
                     A, B = data.A_tensors[k], data.B_tensors[k]
                     if r.model is not None:
-                        _build_smoother_into(data, A, B, r)
+                        _build_smoother_into(data, A, B, r, st.model)
                     else:
                         _storage_view(A).copy_(st.A, non_blocking=True)
                         if r.N > 1:
                             _storage_view(B).copy_(st.B, non_blocking=True)
                 factorize(data)
-                xs = [[] for _ in reqs]
-                solve_ms = [[] for _ in reqs]
+                col_ms = [[] for _ in reqs]     # per column: its solve's device time
                 finished = [0.0] * len(reqs)
-                rounds = max(len(r.rhs) for r in reqs)
+                rounds = max((r.width + P - 1) // P for r in reqs)
                 for j in range(rounds):
-                    live = [k for k, r in enumerate(reqs) if j < len(r.rhs)]
+                    c0 = j * P
+                    live = [k for k, r in enumerate(reqs) if c0 < r.width]
                     for k in live:
-                        _storage_view(data.d_tensors[k]).copy_(staged[k].d[j], non_blocking=True)
+                        c = min(P, reqs[k].width - c0)
+                        dst = _storage_view(data.d_tensors[k])      # (N, P, n)
+                        dst[:, :c].copy_(staged[k].d[:, c0:c0 + c], non_blocking=True)
+                        if c < P:  # spare columns: copies of the first
+                            dst[:, c:].copy_(dst[:, :1].expand(-1, P - c, -1))
                     e0 = torch.cuda.Event(enable_timing=True)
                     e1 = torch.cuda.Event(enable_timing=True)
                     e0.record(stream)
                     solve(data)
                     e1.record(stream)
                     for k in live:
-                        staged[k].out[j].copy_(_storage_view(data.d_tensors[k]), non_blocking=True)
+                        c = min(P, reqs[k].width - c0)
+                        staged[k].out[:, c0:c0 + c].copy_(
+                            _storage_view(data.d_tensors[k])[:, :c], non_blocking=True)
                     stream.synchronize()
                     ms = e0.elapsed_time(e1)
                     now = time.perf_counter() - t0
                     for k in live:
-                        solve_ms[k].append(ms)
-                        xs[k].append(np.transpose(staged[k].out[j].numpy(), (2, 1, 0)).copy())
+                        col_ms[k] += [ms] * min(P, reqs[k].width - c0)
                         finished[k] = now
                 info = data.info()
             finally:
                 data.close()
-        return [Result(r.rid, xs[k], r.arrival, admitted, finished[k], solve_ms[k], int(info[k]))
-                for k, r in enumerate(reqs)]
+        out = []
+        for k, r in enumerate(reqs):
+            w = r.nrhs
+            # right-hand side q: columns [q w, (q + 1) w), as an (n, nrhs, N) view
+            xs = [np.transpose(staged[k].out[:, q * w:(q + 1) * w].numpy(), (2, 1, 0))
+                  for q in range(len(r.rhs))]
+            ms = [col_ms[k][q * w] for q in range(len(r.rhs))]
+            out.append(Result(r.rid, xs, r.arrival, admitted, finished[k], ms, int(info[k])))
+        return out
 
     # -- the loop ----------------------------------------------------------------------
     def serve(self, requests: List[Request], realtime: bool = True) -> List[Result]:
@@ -273,7 +333,7 @@ class Server:
         available at once (a closed-loop throughput run)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
-            fp = self.footprint(r.N, r.n, r.nrhs)
+            fp = self.footprint(r)
             if fp > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs {fp} bytes, more than the HBM "
                                  f"budget of {self.hbm_budget}")
@@ -307,13 +367,13 @@ class Server:
             # coalesced with the waiting requests of the same shape that still fit
             while fifo and free_streams:
                 head = fifo[0]
-                if used + self.footprint(head.N, head.n, head.nrhs) > self.hbm_budget:
+                if used + self.footprint(head) > self.hbm_budget:
                     break
                 group = [head]
                 for r in fifo[1:]:
                     if len(group) >= self.max_group:
                         break
-                    if r.n == head.n and r.nrhs == head.nrhs and used + self._group_fp(
+                    if self._same_class(r, head) and used + self._group_fp(
                             group + [r]) <= self.hbm_budget:
                         group.append(r)
                 fp = self._group_fp(group)
@@ -357,6 +417,41 @@ class Server:
         for th in threads:
             th.join()
         return [results[r.rid] for r in requests]
+
+
+def group_plan_rate(server: Server, requests: List[Request], repeats: int = 3) -> float:
+    """Systems/s of `requests` all present at once and fully coalesced: one group plan
+    per block size and column class, each run by the server's group code on its own
+    stream and thread (staging, blocks, factorisation, solves and transfers as served).
+    The rate the request loop approaches at saturation; best of `repeats` runs."""
+    classes: Dict[Tuple[int, bool], List[Request]] = {}
+    for r in requests:
+        classes.setdefault((r.n, r.width > 1), []).append(r)
+    staged = {r.rid: _Staged(r, server.dtype) for r in requests}
+    groups = list(classes.values())
+    streams = [torch.cuda.Stream(server.device) for _ in groups]
+    best = 0.0
+    for _ in range(repeats):
+        torch.cuda.synchronize(server.device)
+        t0 = time.perf_counter()
+        errs: List[BaseException] = []
+
+        def run(g, s):
+            try:
+                server._run(g, [staged[r.rid] for r in g], s, t0, 0.0)
+            except BaseException as exc:  # re-raised below
+                errs.append(exc)
+
+        th = [threading.Thread(target=run, args=(g, s)) for g, s in zip(groups, streams)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if errs:
+            raise errs[0]
+        torch.cuda.synchronize(server.device)
+        best = max(best, len(requests) / (time.perf_counter() - t0))
+    return best
 
 
 def summarize(results: List[Result]) -> dict:
@@ -406,8 +501,12 @@ def smoother_requests(count: int, rate: float, n_choices=(16, 32, 64), N_range=(
 def solve_alone(req: Request, mode: str = "plain", dtype=torch.float64, device="cuda",
                 **plan_options) -> List[np.ndarray]:
     """The same request on its own plan, on the default stream, nothing else in
-    flight: the reference point for served answers."""
-    data = initialize_batched(req.N, req.n, dtype, device, nrhs=req.nrhs, mode=mode,
+    flight: the reference point for served answers.  The columns are solved as the
+    server solves them, ``solve_width(req.width)`` at a time (spare columns of the
+    last round filled with copies of its first)."""
+    W = req.width
+    P = solve_width(W)
+    data = initialize_batched(req.N, req.n, dtype, device, nrhs=P, mode=mode,
                               **plan_options)
     try:
         npt = np.float64 if dtype == torch.float64 else np.float32
@@ -418,12 +517,18 @@ def solve_alone(req: Request, mode: str = "plain", dtype=torch.float64, device="
             if req.N > 1:
                 data.B_tensor.copy_(torch.from_numpy(req.B.astype(npt)))
         factorize(data)
-        xs = []
-        for d in req.rhs:
-            data.d_tensor.copy_(torch.from_numpy(d.astype(npt)))
+        cols = np.concatenate([d.astype(npt) for d in req.rhs], axis=1)  # (n, W, N)
+        x = np.empty_like(cols)
+        for c0 in range(0, W, P):
+            c = min(P, W - c0)
+            blk = np.empty((req.n, P, req.N), dtype=npt, order="F")
+            blk[:, :c] = cols[:, c0:c0 + c]
+            blk[:, c:] = cols[:, c0:c0 + 1]
+            data.d_tensor.copy_(torch.from_numpy(blk))
             solve(data)
             torch.cuda.synchronize()
-            xs.append(data.d_tensor.cpu().numpy().copy())
-        return xs
+            x[:, c0:c0 + c] = data.d_tensor.cpu().numpy()[:, :c]
+        w = req.nrhs
+        return [np.asfortranarray(x[:, q * w:(q + 1) * w]) for q in range(len(req.rhs))]
     finally:
         data.close()

@@ -43,9 +43,11 @@ def test_admission_respects_hbm_budget():
     """With a budget that fits only the two largest requests, the loop never holds
     more than the budget in flight and still serves everything."""
     reqs = small_stream(10, rate=0.0, seed=5)  # all arrive at once
-    fps = sorted(serving.request_footprint(r.N, r.n, r.nrhs) for r in reqs)
+    fps = sorted(serving.request_footprint(r.N, r.n, serving.solve_width(r.width))
+                 for r in reqs)
     budget = fps[-1] + fps[-2]
     srv = serving.Server(workers=8, hbm_budget=budget, tuned=False)
+    assert sorted(srv.footprint(r) for r in reqs) == fps
     res = srv.serve(reqs, realtime=False)
     assert srv.stats["max_inflight_bytes"] <= budget
     assert len(res) == len(reqs)
@@ -109,8 +111,8 @@ def test_serve_frames_round_trip():
 
 @pytest.mark.parametrize("mode", ["plain", "mixed"])
 def test_coalesced_groups_equal_alone(mode):
-    """Requests waiting together are coalesced into group plans (same n and right-hand-
-    side width; one factorisation and one solve per round for the whole group), and
+    """Requests waiting together are coalesced into group plans (same n and column
+    class; one factorisation and one solve of all columns for the whole group), and
     every answer still equals the request solved alone."""
     reqs = small_stream(16, rate=0.0, seed=31, device_blocks=True)  # all arrive at once
     srv = serving.Server(mode=mode, workers=2, tuned=False)
@@ -124,3 +126,61 @@ def test_coalesced_groups_equal_alone(mode):
                 assert np.array_equal(xs, xa), (r.rid, k)
             else:
                 assert np.abs(xs - xa).max() <= 1e-12 * np.abs(xa).max(), (r.rid, k)
+
+
+@pytest.mark.parametrize("T", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [16, 64, 128])
+def test_columns_do_not_depend_on_width(n, T):
+    """The multi-column sweeps compute every column on its own: the first two columns
+    get the same bits from solves 2, 3, 8 and 16 columns wide.  The server relies on
+    this when it solves a request inside a wider group plan."""
+    from blockstructuredsolvers_amd import factorize, initialize_batched, solve, workloads
+    N = 150
+    npt = np.float64 if T == torch.float64 else np.float32
+    sysm = workloads.spd_system(N, n, nrhs=16, seed=5 + n)
+    outs = []
+    for P in (2, 3, 8, 16):
+        data = initialize_batched(N, n, T, "cuda", nrhs=P, tuned=False)
+        data.A_tensor.copy_(torch.from_numpy(sysm.A.astype(npt)))
+        data.B_tensor.copy_(torch.from_numpy(sysm.B.astype(npt)))
+        data.d_tensor.copy_(torch.from_numpy(np.asfortranarray(sysm.d[:, :P].astype(npt))))
+        factorize(data)
+        solve(data)
+        torch.cuda.synchronize()
+        outs.append(data.d_tensor.cpu().numpy()[:, :2].copy())
+        data.close()
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_wide_requests_and_padding(mode):
+    """Requests of 1 to 20 columns (several right-hand sides of 1 to 3 columns each):
+    one-column requests in one-column plans, wider ones padded into plans up to 16
+    columns wide and solved in rounds of 16 beyond that; every answer equals the
+    request solved alone and meets the oracle's backward-error bar."""
+    rng = np.random.default_rng(41)
+    reqs = []
+    for k, (reps, w) in enumerate([(1, 1), (1, 1), (2, 1), (3, 2), (1, 3), (7, 1), (5, 3),
+                                   (4, 3), (1, 1), (6, 3)]):
+        base = serving.smoother_requests(1, 0.0, n_choices=(32,), N_range=(60, 200),
+                                         rhs_range=(reps, reps), nrhs=w,
+                                         seed=int(rng.integers(1 << 30)))[0]
+        base.rid = k
+        reqs.append(base)
+    assert max(r.width for r in reqs) > 16
+    srv = serving.Server(mode=mode, workers=2, tuned=False)
+    res = srv.serve(reqs, realtime=False)
+    assert srv.stats["groups"] < len(reqs)
+    for r, out in zip(reqs, res):
+        assert out.info == 0 and len(out.x) == len(r.rhs) and len(out.solve_ms) == len(r.rhs)
+        alone = serving.solve_alone(r, mode=mode, tuned=False)
+        A, B = r.dense()
+        for q, (xs, xa) in enumerate(zip(out.x, alone)):
+            assert xs.shape == (r.n, r.nrhs, r.N)
+            if mode == "plain":
+                assert np.array_equal(xs, xa), (r.rid, q)
+            else:
+                assert np.abs(xs - xa).max() <= 1e-12 * np.abs(xa).max(), (r.rid, q)
+            berr = oracle.backward_error(A, B, np.asfortranarray(xs), r.rhs[q]).max()
+            assert berr <= (1e-13 if mode == "plain" else 1e-12), (r.rid, q, berr)
