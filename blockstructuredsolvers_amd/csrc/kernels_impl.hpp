@@ -3,7 +3,7 @@
 #pragma once
 
 #include "launch.hpp"
-#include "chain_factor_big.hpp"
+#include "chain_factor_staged.hpp"
 #include "level_kernels.hpp"
 
 namespace bss {
@@ -51,7 +51,7 @@ static hipError_t factor_nb(const FactorLevelArgs& a, int nchains, hipStream_t s
 template <typename Tin, typename T, int NB, int NW>
 static hipError_t factor_big_nb(const FactorLevelArgs& a, int nchains, hipStream_t s) {
   if (a.fwd) return hipErrorInvalidValue;  // no fused forward sweep (factor_fuses_forward)
-  hipLaunchKernelGGL((chain_factor_big_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
+  hipLaunchKernelGGL((chain_factor_staged_kernel<Tin, T, NB, NW>), dim3(nchains), dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 

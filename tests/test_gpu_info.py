@@ -111,7 +111,7 @@ def test_info_batch_and_group(mode):
 
 @pytest.mark.parametrize("n", [96, 128])
 def test_info_big_blocks(n):
-    """The triangle-in-LDS factor kernel (n = 96, 128) through the same re-check."""
+    """The staged factor kernel (n = 96, 128) through the same re-check."""
     N = 300
     base = workloads.spd_system(N, n, seed=80 + n)
     pos = plant_positions(N, n, 0)

@@ -95,7 +95,7 @@ def test_f64_multi_rhs16(n):
 
 @pytest.mark.parametrize("n", [96, 128])
 def test_big_blocks_multilevel_and_info(n):
-    """n = 96 / 128 (triangle-in-LDS factor kernel): several levels, chain lengths
+    """n = 96 / 128 (staged factor kernel): several levels, chain lengths
     1..4, a non-SPD block reported at its index."""
     sysm = workloads.spd_system(700, n, nrhs=3, seed=40 + n)
     for cl in (1, 3, 4):
