@@ -107,19 +107,44 @@ __device__ inline void gtile_store(BSS_GLOBAL T* p, int ld, const typename Mfma<
 // (put(regs, buf)) and multiplied after one barrier (mul(s, buf)).  When a wave passes
 // the barrier of step s, every wave has finished mul(s - 1), so buffer (s + 1) & 1 is
 // free for the next put.  The closing barrier leaves both buffers free.
-template <typename Regs, typename Load, typename Put, typename Mul>
+// With one buffer (NBUF = 1) a second barrier per step waits for every mul(s - 1)
+// before the buffer is overwritten.
+template <typename Regs, int NBUF, typename Load, typename Put, typename Mul>
 __device__ __attribute__((always_inline)) inline void staged_steps(int steps, Load&& load,
                                                                    Put&& put, Mul&& mul) {
   Regs r;
   if (steps > 0) load(0, r);
-  for (int s = 0; s < steps; ++s) {
-    put(r, s & 1);
+  auto step = [&](int s) {
+    const int buf = NBUF == 1 ? 0 : (s & 1);
+    if (NBUF == 1 && s > 0) __syncthreads();
+    put(r, buf);
     __syncthreads();
     if (s + 1 < steps) load(s + 1, r);
-    mul(s, s & 1);
+    mul(s, buf);
+  };
+  if constexpr (NBUF == 1) {
+    // the single-buffered variants run at 128 registers a wave: an unrolled step loop
+    // spills inside the steps (28.9 against 31.2 ms for the level-0 FP32 factorisation
+    // of 16 systems at n = 128)
+#pragma unroll 1
+    for (int s = 0; s < steps; ++s) step(s);
+  } else {
+    for (int s = 0; s < steps; ++s) step(s);
   }
   __syncthreads();
 }
+
+// Staging buffers and workgroups per CU.  FP32 factors single-buffer (66 KB of LDS at
+// n = 128) so that two workgroups share a CU and one's potrf and trtri overlap the
+// other's products, within 128 registers a wave (level-0 factorisation of 16 systems
+// at n = 128: 28.9 ms against 37.2 ms with one double-buffered workgroup per CU).  FP64
+// factors double-buffer with one workgroup per CU: at n = 96 two workgroups would fit
+// in LDS, but the 128-register cap spills (41.1 ms against 34.0 ms).
+template <typename T, int NB>
+struct StagedCfg {
+  static constexpr int NBUF = sizeof(T) == 4 ? 1 : 2;
+  static constexpr int WG_PER_CU = NBUF == 2 ? 1 : 2;
+};
 
 // Quads per thread of one staged operand: a 16-row slice (4 quads per column) of an
 // n-column block, or a pair of 16-column panels (n / 4 quads per column)
@@ -137,7 +162,8 @@ struct QuadRegs {
 };
 
 template <typename Tin, typename T, int NB, int NW>
-__global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLevelArgs p) {
+__global__ void __launch_bounds__(64 * NW, (StagedCfg<T, NB>::WG_PER_CU * NW / 4))
+    chain_factor_staged_kernel(FactorLevelArgs p) {
   typedef StagedTri<T, NB> Tri;
   constexpr int TT = Tri::TT, NS = Tri::NS, LDT = Tri::LDT, TS = Tri::TS;
   constexpr int NT = 64 * NW;
@@ -152,7 +178,8 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   typedef typename Mfma<T>::acc_t acc_t;
 
   __shared__ __attribute__((aligned(16))) T tri[NS * TS];
-  __shared__ __attribute__((aligned(16))) T stg[2 * 2 * OP];  // 2 buffers x 2 operands
+  constexpr int NBUF = StagedCfg<T, NB>::NBUF;
+  __shared__ __attribute__((aligned(16))) T stg[NBUF * 2 * OP];  // buffers x 2 operands
   __shared__ int fail;
 
   const ChainRec cd = p.chains[blockIdx.x];
@@ -168,13 +195,17 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   BSS_GLOBAL T* W = gptr((T*)S.W);
   const int w = wave_id();
   const int tid = threadIdx.x;
+  // thread index of the staging quads, re-derived for every block (opaque_v) so that
+  // the per-thread staging addresses are computed where they are used instead of being
+  // hoisted out of the block loop and held in registers for the whole chain
+  int tq = tid;
 
   // k-slice staging: quad qi = (column qi / 4, rows 4 (qi % 4) ..) of rows 16 kt .. of
   // an n x n column-major block, into tile (column / 16) of an operand area
   auto slice_load = [&](auto& v, const auto* M, int kt) {
 #pragma unroll
     for (int j = 0; j < Sh::SLICE_QPT; ++j) {
-      const int qi = tid + NT * j;
+      const int qi = tq + NT * j;
       if (Sh::SLICE_Q % NT == 0 || qi < Sh::SLICE_Q)
         quad_load(v[j], M + (qi >> 2) * NB + 16 * kt + 4 * (qi & 3));
     }
@@ -182,7 +213,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   auto slice_put = [&](T* area, const auto& v) {
 #pragma unroll
     for (int j = 0; j < Sh::SLICE_QPT; ++j) {
-      const int qi = tid + NT * j;
+      const int qi = tq + NT * j;
       const int c = qi >> 2;
       if (Sh::SLICE_Q % NT == 0 || qi < Sh::SLICE_Q)
         quad_put(area + (c >> 4) * TS + (c & 15) * LDT + 4 * (qi & 3), v[j]);
@@ -193,7 +224,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   auto panel_load = [&](auto& v, const auto* M, int c0) {
 #pragma unroll
     for (int j = 0; j < Sh::PANEL_QPT; ++j) {
-      const int qi = tid + NT * j;
+      const int qi = tq + NT * j;
       if (Sh::PANEL_Q % NT == 0 || qi < Sh::PANEL_Q)
         quad_load(v[j], M + (c0 + qi / (NB / 4)) * NB + 4 * (qi % (NB / 4)));
     }
@@ -201,7 +232,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   auto panel_put = [&](T* area, const auto& v) {
 #pragma unroll
     for (int j = 0; j < Sh::PANEL_QPT; ++j) {
-      const int qi = tid + NT * j;
+      const int qi = tq + NT * j;
       const int cc = qi / (NB / 4), m = qi % (NB / 4);
       if (Sh::PANEL_Q % NT == 0 || qi < Sh::PANEL_Q)
         quad_put(area + ((cc >> 4) * TT + (m >> 2)) * TS + (cc & 15) * LDT + 4 * (m & 3), v[j]);
@@ -225,6 +256,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
   }
 
   for (int i = 0; i < len; ++i) {
+    tq = opaque_v(tid);
     const long long blk = start + i;
     const bool last = (i == len - 1);
     const bool hasC = !last || hasR;
@@ -245,7 +277,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
       }
       if (i > 0) {
         const BSS_GLOBAL T* Cp = C + (blk - 1) * NN;
-        staged_steps<QuadRegs<T, Sh::SLICE_QPT>>(
+        staged_steps<QuadRegs<T, Sh::SLICE_QPT>, NBUF>(
             TT, [&](int kt, auto& r) { slice_load(r.v, Cp, kt); },
             [&](const auto& r, int buf) { slice_put(stg + buf * 2 * OP, r.v); },
             [&](int kt, int buf) {
@@ -384,13 +416,13 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
     };
     if (hasC) {
       const BSS_GLOBAL Tin* Bb = B + blk * NN;
-      staged_steps<QuadRegs<Tin, Sh::PANEL_QPT>>(
+      staged_steps<QuadRegs<Tin, Sh::PANEL_QPT>, NBUF>(
           TT / 2, [&](int pair, auto& r) { panel_load(r.v, Bb, 32 * pair); },
           [&](const auto& r, int buf) { panel_put(stg + buf * 2 * OP, r.v); },
           [&](int pair, int buf) { solve_mul(Cb, pair, buf); });
     }
     if (hasL) {
-      staged_steps<QuadRegs<T, Sh::PANEL_QPT>>(
+      staged_steps<QuadRegs<T, Sh::PANEL_QPT>, NBUF>(
           TT / 2, [&](int pair, auto& r) { panel_load(r.v, Wb, 32 * pair); },
           [&](const auto& r, int buf) { panel_put(stg + buf * 2 * OP, r.v); },
           [&](int pair, int buf) { solve_mul(Wb, pair, buf); });
@@ -401,7 +433,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
       acc_t Z[ZQ];
 #pragma unroll
       for (int q = 0; q < ZQ; ++q) Z[q] = Mfma<T>::zero();
-      staged_steps<QuadRegs<T, 2 * Sh::SLICE_QPT>>(
+      staged_steps<QuadRegs<T, 2 * Sh::SLICE_QPT>, NBUF>(
           TT,
           [&](int kt, auto& r) {
             slice_load(*(T(*)[Sh::SLICE_QPT][4])r.v, Cb, kt);
@@ -448,7 +480,7 @@ __global__ void __launch_bounds__(64 * NW) chain_factor_staged_kernel(FactorLeve
       for (int q = 0; q < SQ; ++q) R[q] = Mfma<T>::zero();
 #pragma unroll
       for (int q = 0; q < ZQ; ++q) Bp[q] = Mfma<T>::zero();
-      staged_steps<QuadRegs<T, 2 * Sh::SLICE_QPT>>(
+      staged_steps<QuadRegs<T, 2 * Sh::SLICE_QPT>, NBUF>(
           TT,
           [&](int kt, auto& r) {
             if (hasR) slice_load(*(T(*)[Sh::SLICE_QPT][4])r.v, Cb, kt);
