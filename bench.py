@@ -415,14 +415,16 @@ def batch_line(args, torch, oracle):
 def serve_lines(args):
     """Served stream (Poisson arrivals of smoother systems, n in {16, 32, 64}, N ~
     U[128, 4096], 1-8 right-hand sides per factorisation, blocks built on the device):
-    throughput and p50/p99 latency at a light and a saturating rate, plain and mixed.
-    The saturating rate's line also carries the group-plan rate of the same requests
-    (all present at once, one group plan per block size and column class)."""
+    throughput and p50/p99 latency at a light and a saturating arrival rate and with
+    every request present at t = 0 (rate 0), plain and mixed.  The rate-0 line (else the
+    highest rate's) also carries the group-plan rate of the same requests (all present
+    at once, one group plan per block size and column class)."""
     from blockstructuredsolvers_amd import serve_bench
     out = []
+    sat = 0.0 if 0.0 in args.serve_rates else max(args.serve_rates)
     for mode in ("plain", "mixed"):
         for rate in args.serve_rates:
-            top = rate == max(args.serve_rates)
+            top = rate == sat
             r = serve_bench.run(rate, args.serve_count, mode, workers=8, seed=0, N_max=4096,
                                 group_plan=top)
             keep = {k: r[k] for k in ("mode", "rate_per_s", "requests", "systems_per_s",
@@ -516,7 +518,8 @@ def main():
     ap.add_argument("--serve-count", type=int, default=128,
                     help="served-stream lines: requests per rate and mode (0: skip)")
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
-                    default=[200.0, 3200.0])
+                    default=[200.0, 3200.0, 0.0],
+                    help="served-stream arrival rates per second (0: every request at t = 0)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()

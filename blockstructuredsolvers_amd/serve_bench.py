@@ -21,11 +21,16 @@ from . import serving
 
 def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
         device_blocks: bool = True, max_group: int = 64, group_plan: bool = False) -> dict:
-    """Serve `count` requests arriving at `rate`/s; with ``group_plan`` also the rate of
-    the same requests run at once as one group plan per block size and column class
-    (``serving.group_plan_rate``), the saturation bound of the loop."""
-    reqs = serving.smoother_requests(count, rate, N_range=(128, N_max), seed=seed,
-                                     device_blocks=device_blocks)
+    """Serve `count` requests arriving at `rate`/s, or all at t = 0 for ``rate <= 0`` (the
+    requests of the 3,200/s stream, so that the lines compare like with like); with
+    ``group_plan`` also the rate of the same requests run at once as one group plan per
+    block size and column class (``serving.group_plan_rate``), the saturation bound of
+    the loop."""
+    reqs = serving.smoother_requests(count, rate if rate > 0 else 3200.0, N_range=(128, N_max),
+                                     seed=seed, device_blocks=device_blocks)
+    if rate <= 0:
+        for r in reqs:
+            r.arrival = 0.0
     srv = serving.Server(mode=mode, workers=workers, max_group=max_group)
     srv.serve(reqs[:min(4, count)], realtime=False)  # warm-up: kernels loaded, pools primed
     srv.stats.update(groups=0, max_group=0, max_inflight=0, max_inflight_bytes=0)
@@ -46,7 +51,8 @@ def run(rate: float, count: int, mode: str, workers: int, seed: int, N_max: int,
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--rates", default="50,200,800")
+    ap.add_argument("--rates", default="50,200,800",
+                    help="arrival rates per second; 0: every request at t = 0")
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--mode", default="plain", choices=["plain", "mixed"])
     ap.add_argument("--workers", type=int, default=8)
