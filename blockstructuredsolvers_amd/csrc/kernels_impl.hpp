@@ -67,7 +67,7 @@ hipError_t launch_factor(int nb, int waves, const FactorLevelArgs& a, int nchain
       return factor_nb<Tin, T, 64, 4>(a, nchains, s);
     case 96:
       // FP32 factors at n = 96: the four blocks of the LDS kernel fit (154 KB, one
-      // workgroup per CU); FP64 ones need the triangle-in-LDS kernel
+      // workgroup per CU); FP64 ones take the staged kernel (triangle in LDS)
       if constexpr (sizeof(T) == 4)
         return factor_nb<Tin, T, 96, 12>(a, nchains, s);
       else
