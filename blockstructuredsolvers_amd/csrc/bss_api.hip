@@ -594,9 +594,10 @@ int build(bss_plan* p) {
     p->nconv = (int)wr.size();
     img.put_at(p, p->dwiden, wr.data(), wr.size() * sizeof(wr[0]));
     img.put_at(p, p->dnarrow, nr.data(), nr.size() * sizeof(nr[0]));
-    p->host_piv = (unsigned*)bss::pinned_take(sizeof(unsigned) * 3 * p->nsys);
+    // kept from an earlier set-up that failed later on
+    if (!p->host_piv) p->host_piv = (unsigned*)bss::pinned_take(sizeof(unsigned) * 3 * p->nsys);
     if (!p->host_piv) return fail(BSS_E_NOMEM, "pinned host allocation failed");
-    HIP_TRY(hipEventCreateWithFlags(&p->piv_event, hipEventDisableTiming));
+    if (!p->piv_event) HIP_TRY(hipEventCreateWithFlags(&p->piv_event, hipEventDisableTiming));
   }
   for (size_t l = 2; l < p->gl.size(); ++l) fill_tables(p, l, img);
   if (int e = img.upload(p)) return e;
