@@ -11,7 +11,8 @@ around the same plans:
 * ``Server.serve`` releases requests at their arrival times into a FIFO.  The head of
   the FIFO is admitted as soon as (a) a worker is free and (b) its device footprint --
   plan workspace + A, B, d, as the plan reports it before any allocation -- fits in the
-  HBM budget beside the requests in flight.  There is no batch boundary: a request
+  HBM budget beside the requests in flight.  A coalesced group is admitted by the sum of
+  its systems' footprints at the group's width, an upper bound of its group plan's.  There is no batch boundary: a request
   that arrives while others run starts as soon as a slot and memory free up.
 * Requests are coalesced: when a worker is free, the FIFO head is admitted together
   with every other waiting request of the same block size n and column class (up to
@@ -258,11 +259,14 @@ class Server:
         """Device bytes request `r` holds when it runs alone."""
         return request_footprint(r.N, r.n, solve_width(r.width), self.dtype, self.mode)
 
-    def _group_fp(self, group: List[Request]) -> int:
-        if len(group) == 1:
-            return self.footprint(group[0])
-        return group_footprint([r.N for r in group], group[0].n, group_width(group),
-                               self.dtype, self.mode)
+    def _group_fp(self, group: List[Request], width: Optional[int] = None) -> int:
+        """Admission bound of a group: the sum of its systems' single-system footprints
+        at the group's plan width.  A group plan lays out each system's pieces as a
+        single-system plan does and rounds the pieces it shares once, so its workspace
+        (``group_footprint``) never exceeds this sum, and the bound needs no plan layout
+        (cached per N and width)."""
+        P = group_width(group) if width is None else width
+        return sum(request_footprint(r.N, r.n, P, self.dtype, self.mode) for r in group)
 
     @staticmethod
     def _same_class(a: Request, b: Request) -> bool:
@@ -370,13 +374,19 @@ class Server:
                 if used + self.footprint(head) > self.hbm_budget:
                     break
                 group = [head]
+                P = group_width(group)
+                fp = self._group_fp(group, P)
                 for r in fifo[1:]:
                     if len(group) >= self.max_group:
                         break
-                    if self._same_class(r, head) and used + self._group_fp(
-                            group + [r]) <= self.hbm_budget:
+                    if not self._same_class(r, head):
+                        continue
+                    P2 = max(P, solve_width(r.width))
+                    fp2 = (fp + request_footprint(r.N, r.n, P, self.dtype, self.mode)
+                           if P2 == P else self._group_fp(group + [r], P2))
+                    if used + fp2 <= self.hbm_budget:
                         group.append(r)
-                fp = self._group_fp(group)
+                        P, fp = P2, fp2
                 ids = {r.rid for r in group}
                 fifo = [r for r in fifo if r.rid not in ids]
                 used += fp

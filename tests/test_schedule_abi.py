@@ -206,6 +206,28 @@ def test_serving_workload_and_footprint():
     assert f(100, 32, 1, mode="mixed") < f(200, 32, 1, mode="mixed") < f(200, 64, 1, mode="mixed")
 
 
+def test_group_footprint_bounded_by_sum():
+    """The request loop admits a coalesced group by the sum of its systems' single-system
+    footprints at the group's width, without laying out the group plan.  That sum must
+    bound the group plan's own footprint (host-only plan queries), for random groups over
+    every block size, width, precision and mode; one system gives equality."""
+    from blockstructuredsolvers_amd import serving
+    import torch
+    rng = np.random.default_rng(11)
+    for trial in range(120):
+        G = int(rng.integers(1, 24))
+        n = int(rng.choice([16, 32, 48, 64, 96, 128]))
+        nrhs = int(rng.integers(1, 17))
+        mode = ("plain", "mixed")[trial % 2]
+        dt = torch.float32 if (mode == "plain" and trial % 4 == 0) else torch.float64
+        Ns = [int(v) for v in rng.integers(1, 3000, size=G)]
+        g = serving.group_footprint(Ns, n, nrhs, dt, mode)
+        bound = sum(serving.request_footprint(N, n, nrhs, dt, mode) for N in Ns)
+        assert g <= bound, (Ns, n, nrhs, mode, dt)
+        if G == 1:
+            assert g == bound
+
+
 @pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("N", [2, 3, 5, 9, 17, 50, 100, 257, 2000, 8192, 10007])
 def test_every_separator_has_one_owner(N, schedule):
