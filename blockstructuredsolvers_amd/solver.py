@@ -423,7 +423,9 @@ class BlockTriDiagData_batch:
 
     def __init__(self, Ns: List[int], ns: List[int], T, device, nrhs: int, mode: str,
                  schedule: str, graph: bool, tol: Optional[float],
-                 max_iters: Optional[int], preserve_input: bool, fallback: Optional[bool]):
+                 max_iters: Optional[int], preserve_input: bool, fallback: Optional[bool],
+                 chain_len_levels: Optional[List[int]] = None,
+                 base_max: Optional[int] = None):
         if len(Ns) != len(ns) or not Ns:
             raise ValueError("Ns and ns must name the same, non-zero number of systems")
         self.Ns_user = [int(v) for v in Ns]
@@ -450,6 +452,10 @@ class BlockTriDiagData_batch:
             opts.append((L.BSS_OPT_MAX_ITERS, float(max_iters)))
         if fallback is not None:
             opts.append((L.BSS_OPT_FALLBACK, 1.0 if fallback else 0.0))
+        if base_max is not None:
+            opts.append((L.BSS_OPT_BASE_MAX, float(base_max)))
+        for k, m in enumerate(list(chain_len_levels or [])[:16]):
+            opts.append((L.BSS_OPT_CHAIN_LEN_LEVEL + k, float(m)))
         for opt, v in opts:
             L.call("bss_batch_set_option", b, opt, v)
         I64 = ctypes.c_int64
@@ -530,17 +536,22 @@ def initialize_batch(Ns: List[int], ns: List[int], T=torch.float64, M="cuda", *,
                      nrhs: int = 1, mode: str = "plain", schedule: str = "auto",
                      graph: bool = True, tol: Optional[float] = None,
                      max_iters: Optional[int] = None, preserve_input: bool = True,
-                     fallback: Optional[bool] = None) -> BlockTriDiagData_batch:
+                     fallback: Optional[bool] = None,
+                     chain_len_levels: Optional[List[int]] = None,
+                     base_max: Optional[int] = None) -> BlockTriDiagData_batch:
     """Uneven batch across block sizes (``bss_batch_create``): system s has ``Ns[s]``
     blocks of size ``ns[s]``.  ``factorize``/``solve`` take the returned data like the
-    single-plan data; ``graph`` (default on) runs each call as one hipGraph launch."""
+    single-plan data; ``graph`` (default on) runs each call as one hipGraph launch.
+    ``chain_len_levels``/``base_max`` fix the auto schedule of every system (as for
+    ``initialize_batched``)."""
     if not torch.cuda.is_available():
         raise RuntimeError("initialize_batch needs a HIP device; none is visible")
     device = torch.device(M) if not isinstance(M, torch.device) else M
     if device.type != "cuda":
         raise ValueError(f"M must name a GPU device, got {M!r}")
     return BlockTriDiagData_batch(list(Ns), list(ns), T, device, nrhs, mode, schedule, graph,
-                                  tol, max_iters, preserve_input, fallback)
+                                  tol, max_iters, preserve_input, fallback, chain_len_levels,
+                                  base_max)
 
 
 def factorize(data) -> None:

@@ -46,9 +46,13 @@ def alone(sysm, **kw):
     return x
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_plain_bitwise_alone(graph):
-    data = initialize_batch(NN, NS, torch.float64, "cuda", nrhs=2, graph=graph)
+@pytest.mark.parametrize("graph,sched", [(True, {}), (False, {}),
+                                         (True, {"chain_len_levels": [15] + [3] * 15,
+                                                 "base_max": 4})])
+def test_batch_plain_bitwise_alone(graph, sched):
+    """With the default schedule, and with one fixed for every system (the uneven-batch
+    benchmark's), each answer is bitwise that of the system alone on the same schedule."""
+    data = initialize_batch(NN, NS, torch.float64, "cuda", nrhs=2, graph=graph, **sched)
     for rep in range(2):  # the second round replays the captured graphs on new data
         syss = systems(100 + 10 * rep, 2)
         upload(data, syss)
@@ -59,7 +63,7 @@ def test_batch_plain_bitwise_alone(graph):
         for s, sysm in enumerate(syss):
             N = sysm.A.shape[2]
             x = data.d_tensors[s][:, :, :N].cpu().numpy()
-            assert np.array_equal(x, alone(sysm)), (rep, s, NS[s], N)
+            assert np.array_equal(x, alone(sysm, **sched)), (rep, s, NS[s], N)
             assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-13
 
 
