@@ -43,8 +43,8 @@ def test_admission_respects_hbm_budget():
     """With a budget that fits only the two largest requests, the loop never holds
     more than the budget in flight and still serves everything."""
     reqs = small_stream(10, rate=0.0, seed=5)  # all arrive at once
-    fps = sorted(serving.request_footprint(r.N, r.n, serving.solve_width(r.width))
-                 for r in reqs)
+    fps = sorted(serving.request_footprint(r.N, r.n, serving.solve_width(r.width),
+                                           schedule=serving.SERVE_SCHEDULE) for r in reqs)
     budget = fps[-1] + fps[-2]
     srv = serving.Server(workers=8, hbm_budget=budget, tuned=False)
     assert sorted(srv.footprint(r) for r in reqs) == fps

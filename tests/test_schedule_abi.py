@@ -210,7 +210,8 @@ def test_group_footprint_bounded_by_sum():
     """The request loop admits a coalesced group by the sum of its systems' single-system
     footprints at the group's width, without laying out the group plan.  That sum must
     bound the group plan's own footprint (host-only plan queries), for random groups over
-    every block size, width, precision and mode; one system gives equality."""
+    every block size, width, precision, mode and schedule (the library default and the
+    request loop's); one system gives equality."""
     from blockstructuredsolvers_amd import serving
     import torch
     rng = np.random.default_rng(11)
@@ -220,12 +221,16 @@ def test_group_footprint_bounded_by_sum():
         nrhs = int(rng.integers(1, 17))
         mode = ("plain", "mixed")[trial % 2]
         dt = torch.float32 if (mode == "plain" and trial % 4 == 0) else torch.float64
+        sched = serving.SERVE_SCHEDULE if trial % 3 == 0 else None
         Ns = [int(v) for v in rng.integers(1, 3000, size=G)]
-        g = serving.group_footprint(Ns, n, nrhs, dt, mode)
-        bound = sum(serving.request_footprint(N, n, nrhs, dt, mode) for N in Ns)
-        assert g <= bound, (Ns, n, nrhs, mode, dt)
+        g = serving.group_footprint(Ns, n, nrhs, dt, mode, sched)
+        bound = sum(serving.request_footprint(N, n, nrhs, dt, mode, sched) for N in Ns)
+        assert g <= bound, (Ns, n, nrhs, mode, dt, sched)
         if G == 1:
             assert g == bound
+    # the schedule changes the workspace (fewer, smaller separator levels with long chains)
+    assert (serving.request_footprint(4000, 64, 1, schedule=serving.SERVE_SCHEDULE)
+            != serving.request_footprint(4000, 64, 1))
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
