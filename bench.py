@@ -484,7 +484,7 @@ def main():
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--nsys", type=int, default=16)
+    ap.add_argument("--nsys", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--schedule", default="auto", choices=["auto", "reference"])
     ap.add_argument("--chain-len", type=int, default=None)
