@@ -42,12 +42,78 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // TS (V1 up to n = 64, no ||M|| pass): B_i^T x_i is not read again from L2 as a
 // transposed operand; the products B_i[r][k] x_i[r] of the registers that serve B_i
 // x_{i+1} go to an LDS tile, and the next row sums its columns.
-template <int NB, bool V1, bool TS>
+// RQ (V1 from n = 96 on, where that tile would not fit in LDS): the same products are
+// summed over the 16 rows of each wave in registers (red16) and only the per-wave sums
+// go to LDS, n values per wave; the next row adds the waves' sums.  The ||M|| pass sums
+// |B_i| the same way.  The transposed re-read of B_i it replaces ran this kernel at 3.7
+// TB/s at n = 128 (its strided 32-byte reads), against 5.6 without it.
+template <int KC>
+struct Red16 {
+  // halving steps (masks 8, 4, ..) while the slot count stays even; the rest exchange
+  // whole values
+  static constexpr int halvings() {
+    int v = KC, h = 0;
+    while (h < 4 && v % 2 == 0) {
+      v /= 2;
+      ++h;
+    }
+    return h;
+  }
+  static constexpr int H = halvings();
+  static constexpr int CNT = KC >> H;  // sums each lane holds at the end
+};
+
+// Sums KC values of every lane over the 16 lanes of its row group (lanes l ^ 1, 2, 4,
+// 8).  Halving step with mask m: the lane whose mask bit is clear keeps the first half of
+// its slots, the other the second half, and each adds the partner's copy of the half it
+// keeps; then whole exchanges for the masks left.  On return v[0 .. CNT) of lane lr are
+// the sums of local slots red16_first(lr) .. + CNT (the same in lanes that differ only in
+// the exchanged mask bits).
+template <int KC>
+__device__ inline void red16(double (&v)[KC], int lr) {
+  constexpr int H = Red16<KC>::H;
+  int V = KC;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const int m = 8 >> i;
+    const bool up = (lr & m) != 0;
+    const int half = V / 2;
+#pragma unroll
+    for (int j = 0; j < KC / 2; ++j) {
+      if (j < half) {
+        const double send = up ? v[j] : v[half + j];
+        const double recv = __shfl_xor(send, m);
+        v[j] = (up ? v[half + j] : v[j]) + recv;
+      }
+    }
+    V = half;
+  }
+#pragma unroll
+  for (int i = H; i < 4; ++i) {
+    const int m = 8 >> i;
+#pragma unroll
+    for (int j = 0; j < Red16<KC>::CNT; ++j) v[j] += __shfl_xor(v[j], m);
+  }
+}
+template <int KC>
+__device__ inline int red16_first(int lr) {
+  int f = 0, V = KC;
+#pragma unroll
+  for (int i = 0; i < Red16<KC>::H; ++i) {
+    V /= 2;
+    if (lr & (8 >> i)) f += V;
+  }
+  return f;
+}
+
+// MN: the ||M|| pass may run (taken from mnorm_arg for several columns; one-column
+// kernels are launched with MN equal to it, which keeps branches out of their k loop).
+template <int NB, bool V1, bool TS, bool MN>
 __global__ void __launch_bounds__(4 * NB)
     residual_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
                     double* __restrict__ part, int nrhs, int xo_k, int dx_k, int xn_k, int ro_k,
                     int mnorm_arg) {
-  static_assert(!TS || (V1 && NB <= 64), "transpose-sum path: one column, n <= 64");
+  static_assert(!TS || (V1 && NB <= 64 && !MN), "transpose-sum path: one column, n <= 64");
   constexpr int TT = NB / 16, NT = 64 * TT, LDV = NB + 2, KQ = NB / 4;
   constexpr long long NN = (long long)NB * NB;
   typedef Mfma<double>::acc_t acc_t;
@@ -55,7 +121,10 @@ __global__ void __launch_bounds__(4 * NB)
   __shared__ double red[4][4 * TT][16];
   constexpr int LDP = NB + 1;
   __shared__ double P[TS ? NB * LDP : 1];  // TS: B_i[r][k] x_i[r] at P[r * LDP + k]
-  const int mnorm = TS ? 0 : mnorm_arg;
+  constexpr bool RQ = V1 && !TS && NB >= 96;
+  // RQ: per-wave sums over rows r of B_i[r][k] x_i[r] (Q[0]) and |B_i[r][k]| (Q[1])
+  __shared__ double Q[RQ ? 2 : 1][RQ ? TT * NB : 1];
+  const int mnorm = !MN ? 0 : V1 ? 1 : mnorm_arg;
   const ResItem it = items[blockIdx.x];
   const ResSys& S = systems[it.sys];
   const int N = S.N;
@@ -144,6 +213,9 @@ __global__ void __launch_bounds__(4 * NB)
         if constexpr (!V1) carry = Mfma<double>::mma(aBt[q], Xm[lr * LDV + kof(q)], carry);
         if (mnorm) rs_carry += fabs(aBt[q]);
       }
+      if constexpr (RQ) {
+        if (mnorm) rs_carry = row4_sum(rs_carry);  // RQ keeps whole row sums
+      }
     }
   }
 
@@ -162,27 +234,38 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
     for (int q = 0; q < KQ; ++q) pA[q] = As[b0 * NN + (long long)kof(q) * NB + 16 * w + lr];
   }
+  // RQ runs in two chunks.  EO (explicit order): the second chunk's loads are issued
+  // before the first chunk's work; left to itself the scheduler issues them after it,
+  // and a work-group (one per CU at n = 128) then has half a block row in flight.
+  // At n = 96 the registers this takes cost the second work-group per CU, and the
+  // scheduler's own order measured faster.
+  static_assert(!RQ || NCH == 2, "the carry reduction runs in two chunks");
+  constexpr bool EO = RQ && NB >= 128;
   for (int blk = b0; blk < b1; ++blk) {
     // operands of this block row, issued first; B_blk twice (row blk, and transposed
-    // for row blk + 1 when that row is ours)
+    // for row blk + 1 when that row is ours) unless TS or RQ carries B^T x
     const bool next = blk + 1 < N;
     const bool carry_on = next && blk + 1 < b1;
     double xs[SE];
     stage_load(blk + 1, xs);
     double aA[KC], aB[KC], aBn[KC];
+    double cA[EO ? KC : 1], cB[EO ? KC : 1];  // EO: chunk 0
     // B_blk exists only below the last block; otherwise its loads read A_blk (always
     // there) and are zeroed, so no load is conditional
     const auto Bb = next ? Bs + blk * NN : As + blk * NN;
-    auto load_chunk = [&](int ch) {
+    auto load_ops = [&](int ch, double(&uA)[KC], double(&uB)[KC]) {
 #pragma unroll
       for (int qq = 0; qq < KC; ++qq) {
         const int k = kof(ch * KC + qq);
-        if constexpr (PF) aA[qq] = pA[ch * KC + qq];
-        else aA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
+        if constexpr (PF) uA[qq] = pA[ch * KC + qq];
+        else uA[qq] = As[blk * NN + (long long)k * NB + 16 * w + lr];
         const double b = Bb[(long long)k * NB + 16 * w + lr];
-        aB[qq] = next ? b : 0.0;
+        uB[qq] = next ? b : 0.0;
       }
-      if constexpr (!TS) {
+    };
+    auto load_chunk = [&](int ch) {
+      load_ops(ch, aA, aB);
+      if constexpr (!TS && !RQ) {
 #pragma unroll
         for (int qq = 0; qq < KC; ++qq) {
           const double b = Bb[(long long)(16 * w + lr) * NB + kof(ch * KC + qq)];
@@ -204,7 +287,8 @@ __global__ void __launch_bounds__(4 * NB)
         acc[v] = (lr < nrhs) ? x : 0.0;
       }
     }
-    load_chunk(0);
+    if constexpr (!EO) load_chunk(0);
+    else load_ops(0, cA, cB);
     if constexpr (PF) {
       const int j = blk + 1 < b1 ? blk + 1 : blk;
 #pragma unroll
@@ -228,6 +312,20 @@ __global__ void __launch_bounds__(4 * NB)
         carry1 = row_sum(t);
       }
     }
+    if constexpr (RQ) {
+      // this row's carry (and |B| row sum): the previous row's per-wave sums (written
+      // after the previous row's second barrier, rewritten only after this row's)
+      if (blk > b0) {
+        double t = 0.0, u = 0.0;
+#pragma unroll
+        for (int v = 0; v < TT; ++v) {
+          t += Q[0][v * NB + 16 * w + lr];
+          if (mnorm) u += Q[1][v * NB + 16 * w + lr];
+        }
+        carry1 = t;
+        if (mnorm) rs_carry = u;
+      }
+    }
     stage_store(blk + 1, xs);
     __syncthreads();
     const double* X0 = X[blk % 3];
@@ -247,26 +345,73 @@ __global__ void __launch_bounds__(4 * NB)
       acc_t cn = Mfma<double>::zero();
       // V1 partial sums: A x and B x in two chains, B^T x (the next row's carry)
       [[maybe_unused]] double sa = 0.0, sb = 0.0, sn = 0.0;
-      double rs = rs_carry, rsn = 0.0;
+      double rs = RQ ? 0.0 : rs_carry, rsn = 0.0;
+      // RQ: the next row's carry, B[r][k] x[r] summed over the wave's 16 rows (and
+      // |B[r][k]| for ||M||), to Q[.][w NB + k], for slots q0 .. q0 + G of chunk ch as
+      // their FMAs end (a whole chunk at once spills at n = 128).  No branch (a branch
+      // splits the unrolled loop and its loads then spill): every lane stores, the lanes
+      // that share a sum hold the same bits (its adds are commutative); rows without a
+      // next row in this item store sums nobody reads.
+      constexpr int G = KC % 8 == 0 ? 8 : 4;
+      auto rq_sums = [&](int ch, int q0, const double(&uB)[KC]) {
+        constexpr int CNT = Red16<G>::CNT;
+        const int f = red16_first<G>(lr);
+        const double xr = X0[16 * w + lr];
+        double t[G];
 #pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        if (ch > 0) load_chunk(ch);
+        for (int j = 0; j < G; ++j) t[j] = uB[q0 + j] * xr;
+        red16<G>(t, lr);
 #pragma unroll
-        for (int qq = 0; qq < KC; ++qq) {
-          const int k = kof(ch * KC + qq);
-          if constexpr (V1) {
-            const double x0 = X0[k];
-            sa = fma(aA[qq], x0, sa);
-            sb = fma(aB[qq], Xp[k], sb);
-            if constexpr (!TS) sn = fma(aBn[qq], x0, sn);
-          } else {
-            acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
-            acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
-            cn = Mfma<double>::mma(aBn[qq], X0[lr * LDV + k], cn);
+        for (int c = 0; c < CNT; ++c) Q[0][w * NB + kof(ch * KC + q0 + f + c)] = t[c];
+        if (mnorm) {
+#pragma unroll
+          for (int j = 0; j < G; ++j) t[j] = fabs(uB[q0 + j]);
+          red16<G>(t, lr);
+#pragma unroll
+          for (int c = 0; c < CNT; ++c) Q[1][w * NB + kof(ch * KC + q0 + f + c)] = t[c];
+        }
+      };
+      if constexpr (EO) {
+        auto chunk_work = [&](int ch, const double(&uA)[KC], const double(&uB)[KC]) {
+#pragma unroll
+          for (int qq = 0; qq < KC; ++qq) {
+            const int k = kof(ch * KC + qq);
+            sa = fma(uA[qq], X0[k], sa);
+            sb = fma(uB[qq], Xp[k], sb);
+            if (mnorm) rs += fabs(uA[qq]) + fabs(uB[qq]);
+            if (qq % G == G - 1) rq_sums(ch, qq + 1 - G, uB);
           }
-          if (mnorm) {
-            rs += fabs(aA[qq]) + fabs(aB[qq]);
-            rsn += fabs(aBn[qq]);
+        };
+        // chunk 1's loads, then chunk 0's work, kept in that order by the barrier
+        double nA[KC], nB[KC];
+        load_ops(1, nA, nB);
+        __builtin_amdgcn_sched_barrier(0);
+        chunk_work(0, cA, cB);
+        chunk_work(1, nA, nB);
+      } else {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+          if (ch > 0) load_chunk(ch);
+#pragma unroll
+          for (int qq = 0; qq < KC; ++qq) {
+            const int k = kof(ch * KC + qq);
+            if constexpr (V1) {
+              const double x0 = X0[k];
+              sa = fma(aA[qq], x0, sa);
+              sb = fma(aB[qq], Xp[k], sb);
+              if constexpr (!TS && !RQ) sn = fma(aBn[qq], x0, sn);
+            } else {
+              acc = Mfma<double>::mma(-aA[qq], X0[lr * LDV + k], acc);
+              acc = Mfma<double>::mma(-aB[qq], Xp[lr * LDV + k], acc);
+              cn = Mfma<double>::mma(aBn[qq], X0[lr * LDV + k], cn);
+            }
+            if (mnorm) {
+              rs += fabs(aA[qq]) + fabs(aB[qq]);
+              if constexpr (!RQ) rsn += fabs(aBn[qq]);
+            }
+            if constexpr (RQ) {
+              if (qq % G == G - 1) rq_sums(ch, qq + 1 - G, aB);
+            }
           }
         }
       }
@@ -277,16 +422,19 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
           for (int q = 0; q < KQ; ++q) P[(16 * w + lr) * LDP + kof(q)] = aB[q] * xr;
         }
+      } else if constexpr (V1 && RQ) {
+        d1 = d1 - carry1 - row_sum(sa + sb);  // the next carry comes through Q
       } else if constexpr (V1) {
         d1 = d1 - carry1 - row_sum(sa + sb);
         carry1 = row_sum(sn);
       } else {
         carry = cn;
       }
-      rs_carry = rsn;
+      if constexpr (!RQ) rs_carry = rsn;
       if (mnorm) {
         // the four lanes sharing lr hold the four k-slices of row 16w + lr
         rs = row4_sum(rs);
+        if constexpr (RQ) rs += rs_carry;  // the |B_{i-1}| column sum, already whole
         mmax = nmax(mmax, rs);
       }
     }
@@ -431,15 +579,17 @@ hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, i
   if (init)
     hipLaunchKernelGGL(init_pass_kernel, dim3(m.nitems), dim3(kInitThreads), 0, s, m.items,
                        m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k);
-  else if (m.nrhs == 1 && NB <= 64 && !want_mnorm)
-    hipLaunchKernelGGL((residual_kernel<NB, true, NB <= 64>), dim3(m.nitems), dim3(4 * NB), 0,
-                       s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, 0);
+  else if (m.nrhs == 1 && !want_mnorm)
+    hipLaunchKernelGGL((residual_kernel<NB, true, NB <= 64, false>), dim3(m.nitems),
+                       dim3(4 * NB), 0, s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k,
+                       ro_k, 0);
   else if (m.nrhs == 1)
-    hipLaunchKernelGGL((residual_kernel<NB, true, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
-                       m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
+    hipLaunchKernelGGL((residual_kernel<NB, true, false, true>), dim3(m.nitems), dim3(4 * NB),
+                       0, s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, 1);
   else
-    hipLaunchKernelGGL((residual_kernel<NB, false, false>), dim3(m.nitems), dim3(4 * NB), 0, s,
-                       m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k, want_mnorm);
+    hipLaunchKernelGGL((residual_kernel<NB, false, false, true>), dim3(m.nitems), dim3(4 * NB),
+                       0, s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k, ro_k,
+                       want_mnorm);
   return hipGetLastError();
 }
 

@@ -464,3 +464,19 @@ def test_mixed_first_pass_check_depends_on_tolerance():
     x, data = run_gpu(sysm, mode="mixed")
     assert data.iterations >= 2
     assert oracle.backward_error(sysm.A, sysm.B, x, sysm.d).max() <= 1e-12
+
+
+@pytest.mark.parametrize("n,nrhs", [(16, 1), (64, 1), (96, 1), (128, 1), (96, 2), (128, 3)])
+def test_mixed_device_backward_error_after_one_pass(n, nrhs):
+    """After one FP32 pass the residual is about 1e-7 of ||M|| ||x|| and FP64 evaluates
+    it to about 1e-9 of itself, so the device's backward error must match the oracle's
+    closely: this pins the ||M||_inf row sums (|A_i|, |B_i|, and the |B_{i-1}^T| column
+    sums that the one-column kernels from n = 96 on reduce across lanes) and the carry
+    B_{i-1}^T x_{i-1} in the residual itself."""
+    sysm = workloads.spd_system(300, n, nrhs=nrhs, seed=700 + n + nrhs)
+    x, data = run_gpu(sysm, mode="mixed", tol=1e-5)
+    assert data.iterations == 1
+    berr = oracle.backward_error(sysm.A, sysm.B, x, sysm.d)
+    dev = data.backward_error()[0]
+    assert berr.max() <= 1e-5
+    assert np.allclose(dev, berr, rtol=1e-6, atol=0), (dev, berr)
