@@ -42,11 +42,13 @@ __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b
 // TS (V1 up to n = 64, no ||M|| pass): B_i^T x_i is not read again from L2 as a
 // transposed operand; the products B_i[r][k] x_i[r] of the registers that serve B_i
 // x_{i+1} go to an LDS tile, and the next row sums its columns.
-// RQ (V1 from n = 96 on, where that tile would not fit in LDS): the same products are
-// summed over the 16 rows of each wave in registers (red16) and only the per-wave sums
-// go to LDS, n values per wave; the next row adds the waves' sums.  The ||M|| pass sums
-// |B_i| the same way.  The transposed re-read of B_i it replaces ran this kernel at 3.7
-// TB/s at n = 128 (its strided 32-byte reads), against 5.6 without it.
+// RQ (V1 from n = 96 on, where that tile would not fit in LDS, and at n = 64 with the
+// ||M|| pass): the same products are summed over the 16 rows of each wave in registers
+// (red16) and only the per-wave sums go to LDS, n values per wave; the next row adds the
+// waves' sums.  The ||M|| pass sums |B_i| the same way.  The transposed re-read of B_i
+// it replaces ran this kernel at 3.7 TB/s at n = 128 (its strided 32-byte reads),
+// against 5.6 without it; at n = 64 the ||M|| residual went from 3.56 to 3.42 ms for
+// 32 systems, while without ||M|| the LDS tile stays as fast (3.29 against 3.32 ms).
 template <int KC>
 struct Red16 {
   // halving steps (masks 8, 4, ..) while the slot count stays even; the rest exchange
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(4 * NB)
   __shared__ double red[4][4 * TT][16];
   constexpr int LDP = NB + 1;
   __shared__ double P[TS ? NB * LDP : 1];  // TS: B_i[r][k] x_i[r] at P[r * LDP + k]
-  constexpr bool RQ = V1 && !TS && NB >= 96;
+  constexpr bool RQ = V1 && !TS && NB >= 64;
   // RQ: per-wave sums over rows r of B_i[r][k] x_i[r] (Q[0]) and |B_i[r][k]| (Q[1])
   __shared__ double Q[RQ ? 2 : 1][RQ ? TT * NB : 1];
   const int mnorm = !MN ? 0 : V1 ? 1 : mnorm_arg;
@@ -234,13 +236,13 @@ __global__ void __launch_bounds__(4 * NB)
 #pragma unroll
     for (int q = 0; q < KQ; ++q) pA[q] = As[b0 * NN + (long long)kof(q) * NB + 16 * w + lr];
   }
-  // RQ runs in two chunks.  EO (explicit order): the second chunk's loads are issued
-  // before the first chunk's work; left to itself the scheduler issues them after it,
+  // RQ runs in two chunks from n = 96 on (one at n = 64).  EO (explicit order): the
+  // second chunk's loads are issued before the first chunk's work; left to itself the scheduler issues them after it,
   // and a work-group (one per CU at n = 128) then has half a block row in flight.
   // At n = 96 the registers this takes cost the second work-group per CU, and the
   // scheduler's own order measured faster.
-  static_assert(!RQ || NCH == 2, "the carry reduction runs in two chunks");
   constexpr bool EO = RQ && NB >= 128;
+  static_assert(!EO || NCH == 2, "explicit order: two chunks");
   for (int blk = b0; blk < b1; ++blk) {
     // operands of this block row, issued first; B_blk twice (row blk, and transposed
     // for row blk + 1 when that row is ours) unless TS or RQ carries B^T x
