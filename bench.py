@@ -135,6 +135,11 @@ def run_path(mode, systems, args, torch, bss):
     kt_all = kernel_totals(data)
     profile(data, False)
     iters = data.iterations if mode == "mixed" else 0
+    # mixed, one right-hand side: bss_factorize runs the first correction's level-0
+    # forward sweep inside its level-0 factor kernel (BSS_OPT_FACTOR_SWEEP, on by default;
+    # d is restored before each factorisation, so solve finds it unchanged)
+    fused_l0 = (mode == "mixed" and nrhs == 1 and n in (16, 32, 48, 64, 96)
+                and not args.graph)
     berr_dev = data.backward_error().max() if mode == "mixed" else None
     x0 = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
 
@@ -187,6 +192,7 @@ def run_path(mode, systems, args, torch, bss):
         "kernels": kt,
         "kernels_all": kt_all,
         "breakdown_steps": breakdown_steps,
+        "fused_l0": fused_l0,
         "levels": data.level_sizes(),
         "x0": x0,
         "combined": combined,
@@ -261,11 +267,17 @@ def roofline_for(res, args, nsys):
     NL = res["levels"][level]
     P = res["levels"][level + 1] if level + 1 < len(res["levels"]) else 0
     blocks = (NL - P) * nsys          # blocks eliminated by one launch of this level
+    fused = bool(res.get("fused_l0")) and level == 0
+    if name == "forward" and fused:
+        # the first correction's level-0 sweep ran inside the factorisation: its launch
+        # leaves at once (one per step), so the rate is over the launches that swept
+        count = max(count - res.get("profiled_steps", 0), 1)
     avg_ms = total / count
     if name == "factor":
         s_in = 8 if level == 0 else sf
-        # algorithmic: read A, B; write U, C (4 N n^2 s, with input/factor sizes)
-        bytes_per = 2 * n * n * s_in + 2 * n * n * sf
+        # algorithmic: read A, B; write U, C (4 N n^2 s, with input/factor sizes); with the
+        # fused level-0 forward sweep also read r and write y (2 n nrhs sf)
+        bytes_per = 2 * n * n * s_in + 2 * n * n * sf + (2 * n * nrhs * sf if fused else 0)
         flops_per = 7.0 / 3.0 * n ** 3
         alg_bytes = blocks * bytes_per
         alg_flops = blocks * flops_per
@@ -288,9 +300,12 @@ def roofline_for(res, args, nsys):
     else:
         alg_bytes = 0
     ach = alg_bytes / (avg_ms * 1e-3) / 1e9
-    return dict(kernel=f"{name} L{level}", bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None, per_launch_ms=avg_ms,
-                launches=count)
+    out = dict(kernel=f"{name} L{level}", bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None, per_launch_ms=avg_ms,
+               launches=count)
+    if name == "factor" and fused:
+        out["with_forward_sweep"] = True  # the level-0 kernel also runs the first sweep
+    return out
 
 
 def factor_gflops(res, args):
@@ -312,6 +327,7 @@ def kernel_rooflines(res, args, nsys):
             continue
         sub = dict(res)
         sub["kernels"] = {(name, level): res["kernels_all"][(name, level)]}
+        sub["profiled_steps"] = res["breakdown_steps"]
         r = roofline_for(sub, args, nsys)
         out[name] = {"per_launch_ms": r["per_launch_ms"], "achieved": r["achieved"],
                      "unit": r["unit"], "frac": r["frac"]}
@@ -618,7 +634,7 @@ def main():
 
     head = results.get("mixed") or results.get("plain")
     plain = results.get("plain")
-    roof = roofline_for(head, args, args.nsys)
+    roof = roofline_for(dict(head, profiled_steps=args.steps), args, args.nsys)
     traffic = load_traffic(roof["kernel"], head["mode"], args)
     if traffic is not None:
         roof["traffic"] = traffic

@@ -39,6 +39,7 @@ BSS_OPT_GRAPH = 7
 BSS_OPT_PROFILE_KINDS = 8
 BSS_OPT_PROFILE_MAX_LEVEL = 9
 BSS_OPT_FALLBACK = 10
+BSS_OPT_FACTOR_SWEEP = 11
 BSS_OPT_CHAIN_LEN_LEVEL = 32
 
 KERNEL_KINDS = {"factor": 0, "assemble_diag": 1, "forward": 2, "assemble_rhs": 3,

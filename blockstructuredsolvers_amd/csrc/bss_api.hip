@@ -60,6 +60,17 @@ __global__ void reset_status_kernel(int* __restrict__ info, int ninfo, int vinfo
   if (i < ninfo) info[i] = vinfo;
   if (piv && i < npiv) piv[i] = vpiv;
 }
+// *changed = 1 if any of the n 64-bit words of a and b differ (grid-stride; the flag
+// was cleared before)
+__global__ void __launch_bounds__(256) words_differ_kernel(const unsigned long long* __restrict__ a,
+                                                           const unsigned long long* __restrict__ b,
+                                                           long long n, int* __restrict__ changed) {
+  bool diff = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    diff |= a[i] != b[i];
+  if (diff) *changed = 1;
+}
 constexpr int kPivNone = 0x7f800000;  // float +inf bits: no pivot seen
 constexpr size_t kNone = ~size_t(0);   // buffer not allocated
 
@@ -179,6 +190,12 @@ struct bss_plan {
   hipEvent_t ev_l0_factored = nullptr, ev_l0_solved = nullptr;
   std::vector<hipEvent_t> ev_level;  // level l factored (levels beyond the first two)
   size_t fwd_done = 0;
+  // BSS_OPT_FACTOR_SWEEP: bss_factorize ran the first solve's x = 0 pass and level-0
+  // forward sweep on d (fsweep_pending) and copied d to mx.dsnap; the next bss_solve
+  // sets *mx.dflag = 1 on the device if d differs from it, and its first pass and
+  // level-0 forward sweep (fsweep_flag, consumed by the first solve32) run only then
+  bool fsweep_on = true, fsweep_pending = false;
+  const int* fsweep_flag = nullptr;
   void drop_graphs() {
     for (hipGraphExec_t* g : {&g_factor, &g_solve[0], &g_solve[1], &g_solve64[0], &g_solve64[1]}) {
       if (*g) (void)hipGraphExecDestroy(*g);
@@ -701,12 +718,14 @@ int ensure_pivots(bss_plan* p, hipStream_t s) {
 // Solve in place in the level-0 vectors of set k (plain: the caller's d; mixed: the
 // FP32 ping-pong vector k).
 template <typename T>
-int solve_levels(bss_plan* p, int k, hipStream_t s, size_t l_begin = 0) {
+int solve_levels(bss_plan* p, int k, hipStream_t s, size_t l_begin = 0,
+                 const int* skip0 = nullptr) {
   const int nb = p->n;
   const size_t L = p->gl.size();
   for (size_t l = l_begin; l < L; ++l) {
     GLevel& G = p->gl[l];
-    const bss::SolveLevelArgs a{G.dchains, G.dtab[k], p->nrhs};
+    bss::SolveLevelArgs a{G.dchains, G.dtab[k], p->nrhs};
+    if (l == 0) a.skip = skip0;
     hipError_t e;
     {
       ProfScope ps(p, BSS_K_FORWARD, (int)l, s);
@@ -868,11 +887,14 @@ bss::MixedSolveHooks mixed_hooks(bss_plan* p) {
   hooks.solve32 = [](void* ctx, int k, hipStream_t st) -> int {
     bss_plan* q = (bss_plan*)ctx;
     const size_t skip = k == 0 ? q->fwd_done : 0;
+    const int* fsweep = k == 0 ? q->fsweep_flag : nullptr;
     q->fwd_done = 0;
+    q->fsweep_flag = nullptr;
     if (q->sep64)
       return run_graph(q, &q->g_solve64[k], st,
                        [q, k](hipStream_t c) { return solve_levels_sep64(q, k, c); });
     if (skip) return solve_levels<float>(q, k, st, skip);
+    if (fsweep) return solve_levels<float>(q, k, st, 0, fsweep);
     return run_graph(q, &q->g_solve[k], st, [q, k](hipStream_t c) { return solve_levels<float>(q, k, c); });
   };
   hooks.prof_begin = [](void* ctx, int kind, hipStream_t st) -> void* {
@@ -999,6 +1021,7 @@ int plan_factor_prepare(bss_plan* p, void* stream) {
   p->last_stream = (hipStream_t)stream;
   p->info_valid = false;
   p->fb_active = p->fb_factored = false;
+  p->fsweep_pending = false;
   return BSS_OK;
 }
 
@@ -1147,6 +1170,10 @@ int bss_set_option(bss_plan_t p, int option, double value) {
     case BSS_OPT_FALLBACK:
       p->fallback_on = value != 0.0;
       return BSS_OK;
+    case BSS_OPT_FACTOR_SWEEP:
+      p->fsweep_on = value != 0.0;
+      if (!p->fsweep_on) p->fsweep_pending = false;
+      return BSS_OK;
     default:
       if (option >= BSS_OPT_CHAIN_LEN_LEVEL && option < BSS_OPT_CHAIN_LEN_LEVEL + 16) {
         if (value < 1 || value > 1 << 20) return fail(BSS_E_ARG, "chain length must be >= 1");
@@ -1236,10 +1263,26 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
 int bss_factorize(bss_plan_t p, void* stream) {
   if (int e = bss::plan_factor_prepare(p, stream)) return e;
   hipStream_t s = (hipStream_t)stream;
-  const int rc = run_graph(p, &p->g_factor, s,
-                           [p](hipStream_t c) { return bss::plan_factor_kernels(p, c); });
+  // BSS_OPT_FACTOR_SWEEP: the first solve's x = 0 pass and, inside the level-0 factor
+  // kernel, its level-0 forward sweep, on the d found here (kept in dsnap)
+  const bool fsweep = p->fsweep_on && p->mode == BSS_MODE_MIXED && !p->use_graph &&
+                      p->mx.dsnap && bss::factor_fuses_forward(p->n, true, p->nrhs);
+  int rc;
+  if (fsweep) {
+    const size_t nd = packed_vector_elems(p);
+    HIP_TRY(hipMemcpyAsync(p->mx.dsnap, p->d, nd * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemsetAsync(p->mx.dflag, 0, sizeof(int), s));
+    if (bss::mixed_first_pass(p->mx, mixed_hooks(p), s) != 0)
+      return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
+    rc = factor_levels<double, float>(p, s, 0, ~size_t(0), true);
+  } else {
+    rc = run_graph(p, &p->g_factor, s,
+                   [p](hipStream_t c) { return bss::plan_factor_kernels(p, c); });
+  }
   if (rc != BSS_OK) return rc;
-  return bss::plan_factor_finish(p, stream);
+  rc = bss::plan_factor_finish(p, stream);
+  if (rc == BSS_OK && fsweep) p->fsweep_pending = true;
+  return rc;
 }
 
 int bss_solve(bss_plan_t p, void* stream) {
@@ -1248,7 +1291,26 @@ int bss_solve(bss_plan_t p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   p->last_stream = s;
   p->fwd_done = 0;
-  if (p->mode == BSS_MODE_MIXED) return run_mixed_solve(p, s, false);
+  if (p->mode == BSS_MODE_MIXED) {
+    const bool fsweep = p->fsweep_pending && !p->fb_active;
+    p->fsweep_pending = false;
+    if (fsweep) {
+      // d unchanged since bss_factorize, bit for bit, or the x = 0 pass and the level-0
+      // forward sweep run again (the flag is read on the device: no host wait)
+      const long long nd = (long long)packed_vector_elems(p);
+      const int grid = (int)std::min<long long>((nd + 255) / 256, 4096);
+      hipLaunchKernelGGL(words_differ_kernel, dim3(grid), dim3(256), 0, s,
+                         (const unsigned long long*)p->d,
+                         (const unsigned long long*)p->mx.dsnap, nd, p->mx.dflag);
+      HIP_TRY(hipGetLastError());
+      p->mx.init_skip = p->mx.dflag;
+      p->fsweep_flag = p->mx.dflag;
+    }
+    const int rc = run_mixed_solve(p, s, false);
+    p->mx.init_skip = nullptr;
+    p->fsweep_flag = nullptr;
+    return rc;
+  }
   return run_graph(p, &p->g_solve[0], s,
                    [p](hipStream_t c) { return bss::plan_solve_kernels(p, c); });
 }
@@ -1264,6 +1326,7 @@ int bss_factorize_solve(bss_plan_t p, void* stream) {
   p->last_stream = s;
   p->factored = false;
   p->fwd_done = 0;
+  p->fsweep_pending = false;
   p->info_valid = false;
   p->fb_active = p->fb_factored = false;
   if (!p->side_stream) {

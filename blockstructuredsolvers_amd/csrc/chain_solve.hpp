@@ -23,6 +23,9 @@ struct SolveLevelArgs {
   const ChainRec* chains;
   const LevelSys* sys;  // U, C, W, d; forward writes rR/rL, backward reads dn (x_sep)
   int nrhs;
+  // forward sweeps: when set and *skip == 0, the sweep already ran (inside the factor
+  // kernel, on the same right-hand side) and the launch does nothing
+  const int* skip = nullptr;
 };
 
 template <typename T>
@@ -137,6 +140,7 @@ struct XOps {
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p) {
+  if (p.skip && *p.skip == 0) return;  // the whole workgroup leaves (uniform)
   constexpr int TT = NB / 16;
   constexpr int NT = 64 * TT;
   constexpr int LDV = NB + VecPad<T>::value;
@@ -241,6 +245,7 @@ __global__ void __launch_bounds__(4 * NB) chain_forward_kernel(SolveLevelArgs p)
 // 16w + lr of a block's vector; the LDS vectors hold only column 0.
 template <typename T, int NB>
 __global__ void __launch_bounds__(4 * NB) chain_forward1_kernel(SolveLevelArgs p) {
+  if (p.skip && *p.skip == 0) return;  // the whole workgroup leaves (uniform)
   constexpr int TT = NB / 16;
   constexpr long long NN = (long long)NB * NB;
   __shared__ T Y[NB];

@@ -539,8 +539,10 @@ constexpr int kInitThreads = 256;
 
 __global__ void __launch_bounds__(kInitThreads)
     init_pass_kernel(const ResItem* __restrict__ items, const ResSys* __restrict__ systems,
-                     double* __restrict__ part, int n, int nrhs, int xn_k, int ro_k) {
+                     double* __restrict__ part, int n, int nrhs, int xn_k, int ro_k,
+                     const int* __restrict__ skip) {
   __shared__ double red[kInitThreads / 64];
+  if (skip && *skip == 0) return;  // the whole workgroup leaves (uniform)
   const ResItem it = items[blockIdx.x];
   const ResSys& S = systems[it.sys];
   const long long ND = (long long)n * nrhs;
@@ -580,7 +582,7 @@ hipError_t launch_residual(const MixedState& m, bool init, int xo_k, int dx_k, i
                            int ro_k, int want_mnorm, hipStream_t s) {
   if (init)
     hipLaunchKernelGGL(init_pass_kernel, dim3(m.nitems), dim3(kInitThreads), 0, s, m.items,
-                       m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k);
+                       m.sys_dev, m.part, NB, m.nrhs, xn_k, ro_k, m.init_skip);
   else if (m.nrhs == 1 && !want_mnorm)
     hipLaunchKernelGGL((residual_kernel<NB, true, NB <= 64, false>), dim3(m.nitems),
                        dim3(4 * NB), 0, s, m.items, m.sys_dev, m.part, m.nrhs, xo_k, dx_k, xn_k,
@@ -638,6 +640,7 @@ size_t MixedState::bytes(const std::vector<int>& Ns, int n, int nrhs) {
   b += 2 * al(nd_all * 8) + 2 * al(nd_all * 4);
   b += al(items * sizeof(ResItem)) + al(S * sizeof(ResSys)) + al(items * kParts * 8);
   b += al(S * 8) + al(S * 16 * 8) + al(S * 16 * 8);
+  if (factor_fuses_forward(n, true, nrhs)) b += al(nd_all * 8) + al(sizeof(int));
   return b;
 }
 
@@ -685,6 +688,12 @@ int MixedState::attach(char* base, const std::vector<int>& Ns_, int n_, int nrhs
   mnorm = (double*)take(nsys * 8);
   dnorm = (double*)take((size_t)nsys * 16 * 8);
   berr = (double*)take((size_t)nsys * 16 * 8);
+  dsnap = nullptr;
+  dflag = nullptr;
+  if (factor_fuses_forward(n, true, nrhs)) {
+    dsnap = (double*)take(nd_all * 8);
+    dflag = (int*)take(sizeof(int));
+  }
   if (hipMemcpy(items, it.data(), it.size() * sizeof(ResItem), hipMemcpyHostToDevice) !=
       hipSuccess) {
     g_mixed_err = "work-item upload failed";
@@ -747,8 +756,11 @@ void mixed_after_factor(MixedState& m) { m.mnorm_stale = true; }
 constexpr double kFirstCheckTol = 1e-9;
 
 int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s) {
-  // x = 0, r = d (as FP32 into v[0]), ||d||
+  // x = 0, r = d (as FP32 into v[0]), ||d||.  With init_skip the kernel may leave at
+  // once; the backward-error kernel after it still runs, on the partial norms the
+  // skipped pass wrote the last time (nothing else writes them in between)
   const hipError_t e = residual_prof(m, h, true, -1, -1, 0, 0, 0, s);
+  m.init_skip = nullptr;
   if (e != hipSuccess) {
     g_mixed_err = hipGetErrorString(e);
     return -1;

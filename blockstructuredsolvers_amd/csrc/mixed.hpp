@@ -44,6 +44,14 @@ struct MixedState {
   double* host_berr = nullptr;  // pinned mirror of berr
   hipEvent_t berr_ready = nullptr;  // recorded after each copy of berr to host_berr
   bool mnorm_stale = true;      // ||M|| must be recomputed (new factorisation)
+  // the next x = 0 pass only: when set and *init_skip == 0 it does nothing (the same
+  // pass already ran on the same d); mixed_first_pass clears it
+  const int* init_skip = nullptr;
+  // one right-hand side with a fused level-0 sweep (factor_fuses_forward): a copy of d
+  // as the factorisation found it, and a flag set when d differs from it (a plan's
+  // BSS_OPT_FACTOR_SWEEP); null otherwise
+  double* dsnap = nullptr;
+  int* dflag = nullptr;
   std::vector<double> last_berr;
 
   // device bytes for systems of sizes Ns (N_use per system)

@@ -117,6 +117,16 @@ enum bss_option {
                                and solve this and every later right-hand side of the
                                factorisation with it, its backward error measured in
                                FP64; 0 = return BSS_NOT_CONVERGED with the last iterate */
+  BSS_OPT_FACTOR_SWEEP = 11, /* mixed, one right-hand side, n <= 96, no graph, plan-owned
+                               workspace: 1 (default) = bss_factorize also runs the first
+                               solve's level-0 forward sweep on the d it finds (inside the
+                               level-0 factor kernel, as bss_factorize_solve does) and keeps
+                               a device copy of d (n nrhs N_use 8 bytes per system, allocated
+                               on first use).  The next bss_solve compares d with that copy
+                               on the device and skips its x = 0 pass and level-0 forward
+                               sweep if every bit is unchanged, else runs them; the answer is
+                               bitwise that of option value 0 either way.  No counterpart in
+                               the reference (its factorize! never reads d). */
   BSS_OPT_CHAIN_LEN_LEVEL = 32 /* + level k (< 16): chain length of level k only
                                   (auto schedule); overrides BSS_OPT_CHAIN_LEN there */
 };
