@@ -327,6 +327,9 @@ class Server:
             data = initialize_batched_group([r.N for r in reqs], n, self.dtype, self.device,
                                             nrhs=P, mode=self.mode, torch_workspace=True,
                                             **self.plan_options)
+            # d is written after the factorisation here (one column round at a time),
+            # so a factor-time first sweep would only be thrown away
+            L.call("bss_set_option", data._plan, L.BSS_OPT_FACTOR_SWEEP, 0.0)
             try:
                 for k, (r, st) in enumerate(zip(reqs, staged)):
                     A, B = data.A_tensors[k], data.B_tensors[k]
@@ -567,6 +570,7 @@ def solve_alone(req: Request, mode: str = "plain", dtype=torch.float64, device="
     P = solve_width(W)
     data = initialize_batched(req.N, req.n, dtype, device, nrhs=P, mode=mode,
                               **_with_schedule(plan_options))
+    L.call("bss_set_option", data._plan, L.BSS_OPT_FACTOR_SWEEP, 0.0)  # d comes later
     try:
         npt = np.float64 if dtype == torch.float64 else np.float32
         if req.model is not None:
