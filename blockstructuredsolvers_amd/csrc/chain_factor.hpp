@@ -612,9 +612,10 @@ __global__ void __launch_bounds__(64 * NW)
       // d of the next block, one element per lane, in flight through this step's
       // products (registers held within this step only)
       T nd[FDQ];
+      const int fl = opaque_v(lane_id());  // (its addresses held across the chain spill)
 #pragma unroll
       for (int q = 0; q < FDQ; ++q) {
-        const int row = lane_id() + 64 * q;
+        const int row = fl + 64 * q;
         nd[q] = (!last && row < NB) ? dg[(long long)(blk1 + 1) * NB + row] : T(0);
       }
       // r = d_blk - C_{blk-1}^T y_{blk-1}
@@ -656,7 +657,7 @@ __global__ void __launch_bounds__(64 * NW)
       if (!last) {
 #pragma unroll
         for (int q = 0; q < FDQ; ++q) {
-          const int row = lane_id() + 64 * q;
+          const int row = fl + 64 * q;
           if (row < NB) Fd[row] = nd[q];
         }
       }
