@@ -31,6 +31,19 @@ def test_library_exports_header_symbols():
     assert set(syms) == set(L.SIGNATURES)
 
 
+def test_python_constants_match_header_enums():
+    """Every BSS_* enumerator of include/bss.h that the Python mirror names has the
+    header's value there (options, statuses, modes, kernel kinds, queries)."""
+    txt = open(os.path.join(ROOT, "include", "bss.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    enums = {k: int(v) for k, v in re.findall(r"\b(BSS_[A-Z0-9_]+)\s*=\s*(-?\d+)", txt)}
+    assert enums["BSS_OPT_FACTOR_SWEEP"] == 11
+    shared = [k for k in enums if hasattr(L, k)]
+    assert len(shared) >= 12
+    for k in shared:
+        assert getattr(L, k) == enums[k], k
+
+
 def test_argument_errors():
     p = ctypes.c_void_p()
     rc = L.lib().bss_plan_create(10, 24, 1, 1, 1, 0, 0, ctypes.byref(p))
