@@ -327,9 +327,26 @@ class Server:
             data = initialize_batched_group([r.N for r in reqs], n, self.dtype, self.device,
                                             nrhs=P, mode=self.mode, torch_workspace=True,
                                             **self.plan_options)
-            # d is written after the factorisation here (one column round at a time),
-            # so a factor-time first sweep would only be thrown away
-            L.call("bss_set_option", data._plan, L.BSS_OPT_FACTOR_SWEEP, 0.0)
+            rounds = max((r.width + P - 1) // P for r in reqs)
+
+            def put_d(j):
+                # column round j of every request that has one into d (spare columns:
+                # copies of the first); returns the requests in the round
+                c0 = j * P
+                live = [k for k, r in enumerate(reqs) if c0 < r.width]
+                for k in live:
+                    c = min(P, reqs[k].width - c0)
+                    dst = _storage_view(data.d_tensors[k])      # (N, P, n)
+                    dst[:, :c].copy_(staged[k].d[:, c0:c0 + c], non_blocking=True)
+                    if c < P:
+                        dst[:, c:].copy_(dst[:, :1].expand(-1, P - c, -1))
+                return live
+
+            # one round: d goes in before the factorisation, whose level-0 kernel then
+            # also runs the first correction's level-0 sweep (mixed, one column); with
+            # several rounds d is written after it and that sweep would be thrown away
+            if rounds > 1:
+                L.call("bss_set_option", data._plan, L.BSS_OPT_FACTOR_SWEEP, 0.0)
             try:
                 for k, (r, st) in enumerate(zip(reqs, staged)):
                     A, B = data.A_tensors[k], data.B_tensors[k]
@@ -339,19 +356,13 @@ class Server:
                         _storage_view(A).copy_(st.A, non_blocking=True)
                         if r.N > 1:
                             _storage_view(B).copy_(st.B, non_blocking=True)
+                first = put_d(0) if rounds == 1 else None
                 factorize(data)
                 col_ms = [[] for _ in reqs]     # per column: its solve's device time
                 finished = [0.0] * len(reqs)
-                rounds = max((r.width + P - 1) // P for r in reqs)
                 for j in range(rounds):
                     c0 = j * P
-                    live = [k for k, r in enumerate(reqs) if c0 < r.width]
-                    for k in live:
-                        c = min(P, reqs[k].width - c0)
-                        dst = _storage_view(data.d_tensors[k])      # (N, P, n)
-                        dst[:, :c].copy_(staged[k].d[:, c0:c0 + c], non_blocking=True)
-                        if c < P:  # spare columns: copies of the first
-                            dst[:, c:].copy_(dst[:, :1].expand(-1, P - c, -1))
+                    live = first if first is not None else put_d(j)
                     e0 = torch.cuda.Event(enable_timing=True)
                     e1 = torch.cuda.Event(enable_timing=True)
                     e0.record(stream)
