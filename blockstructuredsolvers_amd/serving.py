@@ -25,7 +25,10 @@ around the same plans:
 * Each worker thread owns one HIP stream and runs its group end to end: blocks
   uploaded (or built on the device from the model), one ``factorize`` of all systems,
   then one ``solve`` of every request's columns at once (rounds of 16 columns for
-  wider requests), and the downloads.  The factors stay resident between rounds.
+  wider requests), and the downloads.  The factors stay resident between rounds.  A
+  group with a single round uploads its right-hand sides before the factorisation,
+  which in mixed mode with one column then runs the first correction's level-0 sweep
+  (BSS_OPT_FACTOR_SWEEP); a solve's device time then excludes that sweep.
   Work of different workers overlaps on the device (ctypes releases the GIL during
   library calls).  Plans take their workspace from torch's caching allocator, so
   admission and release make no device allocation or free.
