@@ -138,8 +138,7 @@ def run_path(mode, systems, args, torch, bss):
     # mixed, one right-hand side: bss_factorize runs the first correction's level-0
     # forward sweep inside its level-0 factor kernel (BSS_OPT_FACTOR_SWEEP, on by default;
     # d is restored before each factorisation, so solve finds it unchanged)
-    fused_l0 = (mode == "mixed" and nrhs == 1 and n in (16, 32, 48, 64, 96)
-                and not args.graph)
+    fused_l0 = mode == "mixed" and nrhs == 1 and n in (16, 32, 48, 64, 96)
     berr_dev = data.backward_error().max() if mode == "mixed" else None
     x0 = (data.d_tensor if nsys == 1 else data.d_tensor[..., 0]).cpu().numpy()
 

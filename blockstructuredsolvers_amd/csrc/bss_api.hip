@@ -1025,7 +1025,19 @@ int plan_factor_prepare(bss_plan* p, void* stream) {
   return BSS_OK;
 }
 
+bool plan_factor_sweeps(const bss_plan* p) {
+  return p->fsweep_on && p->mode == BSS_MODE_MIXED && p->mx.dsnap &&
+         factor_fuses_forward(p->n, true, p->nrhs);
+}
+
 int plan_factor_kernels(bss_plan* p, hipStream_t c) {
+  if (plan_factor_sweeps(p)) {
+    // BSS_OPT_FACTOR_SWEEP: d kept in dsnap, the first correction's x = 0 pass, and the
+    // level-0 factor kernel running its level-0 forward sweep (kernels only: capturable)
+    if (mixed_snapshot_d(p->mx, c) != 0 || mixed_first_pass(p->mx, mixed_hooks(p), c) != 0)
+      return fail(BSS_E_HIP, "factor-time first pass: %s", mixed_last_error());
+    return factor_levels<double, float>(p, c, 0, ~size_t(0), true);
+  }
   if (p->mode == BSS_MODE_MIXED) return factor_levels<double, float>(p, c);
   if (p->dtype == BSS_F64) return factor_levels<double, double>(p, c);
   return factor_levels<float, float>(p, c);
@@ -1043,6 +1055,8 @@ int plan_factor_finish(bss_plan* p, void* stream) {
     p->sep64 = false;
   }
   p->factored = true;
+  // plan_factor_kernels ran the first correction's level-0 sweep: the next solve checks d
+  p->fsweep_pending = plan_factor_sweeps(p);
   return BSS_OK;
 }
 
@@ -1173,6 +1187,7 @@ int bss_set_option(bss_plan_t p, int option, double value) {
     case BSS_OPT_FACTOR_SWEEP:
       p->fsweep_on = value != 0.0;
       if (!p->fsweep_on) p->fsweep_pending = false;
+      p->drop_graphs();  // the factorisation's launch sequence changes
       return BSS_OK;
     default:
       if (option >= BSS_OPT_CHAIN_LEN_LEVEL && option < BSS_OPT_CHAIN_LEN_LEVEL + 16) {
@@ -1263,26 +1278,10 @@ int bss_plan_bind(bss_plan_t p, void* A, void* B, void* d) {
 int bss_factorize(bss_plan_t p, void* stream) {
   if (int e = bss::plan_factor_prepare(p, stream)) return e;
   hipStream_t s = (hipStream_t)stream;
-  // BSS_OPT_FACTOR_SWEEP: the first solve's x = 0 pass and, inside the level-0 factor
-  // kernel, its level-0 forward sweep, on the d found here (kept in dsnap)
-  const bool fsweep = p->fsweep_on && p->mode == BSS_MODE_MIXED && !p->use_graph &&
-                      p->mx.dsnap && bss::factor_fuses_forward(p->n, true, p->nrhs);
-  int rc;
-  if (fsweep) {
-    const size_t nd = packed_vector_elems(p);
-    HIP_TRY(hipMemcpyAsync(p->mx.dsnap, p->d, nd * 8, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemsetAsync(p->mx.dflag, 0, sizeof(int), s));
-    if (bss::mixed_first_pass(p->mx, mixed_hooks(p), s) != 0)
-      return fail(BSS_E_HIP, "first residual pass: %s", bss::mixed_last_error());
-    rc = factor_levels<double, float>(p, s, 0, ~size_t(0), true);
-  } else {
-    rc = run_graph(p, &p->g_factor, s,
-                   [p](hipStream_t c) { return bss::plan_factor_kernels(p, c); });
-  }
+  int rc = run_graph(p, &p->g_factor, s,
+                     [p](hipStream_t c) { return bss::plan_factor_kernels(p, c); });
   if (rc != BSS_OK) return rc;
-  rc = bss::plan_factor_finish(p, stream);
-  if (rc == BSS_OK && fsweep) p->fsweep_pending = true;
-  return rc;
+  return bss::plan_factor_finish(p, stream);
 }
 
 int bss_solve(bss_plan_t p, void* stream) {

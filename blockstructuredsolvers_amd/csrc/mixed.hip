@@ -755,6 +755,36 @@ void mixed_after_factor(MixedState& m) { m.mnorm_stale = true; }
 // below this tolerance the backward error after the first FP32 pass is not read
 constexpr double kFirstCheckTol = 1e-9;
 
+namespace {
+__global__ void __launch_bounds__(256) snapshot_kernel(const ResSys* __restrict__ systems,
+                                                       double* __restrict__ snap, long long n,
+                                                       int* __restrict__ flag) {
+  const double* __restrict__ d = systems[0].d;  // system 0 starts the packed d
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    snap[i] = d[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+}
+}  // namespace
+
+int mixed_snapshot_d(MixedState& m, hipStream_t s) {
+  if (!m.dsnap || !m.dflag) {
+    g_mixed_err = "no d snapshot in this workspace";
+    return -1;
+  }
+  long long nd = 0;
+  for (int N : m.Ns) nd += (long long)N * m.n * m.nrhs;
+  const int grid = (int)std::min<long long>((nd + 255) / 256, 4096);
+  hipLaunchKernelGGL(snapshot_kernel, dim3(grid), dim3(256), 0, s, m.sys_dev, m.dsnap, nd,
+                     m.dflag);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_mixed_err = hipGetErrorString(e);
+    return -1;
+  }
+  return 0;
+}
+
 int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s) {
   // x = 0, r = d (as FP32 into v[0]), ||d||.  With init_skip the kernel may leave at
   // once; the backward-error kernel after it still runs, on the partial norms the

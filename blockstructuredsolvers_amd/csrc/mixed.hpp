@@ -116,6 +116,10 @@ inline double* mixed_iterate(const MixedState& m, int k) { return m.sys_host[0].
 // starts at the first FP32 solve.
 int mixed_first_pass(MixedState& m, const MixedSolveHooks& h, hipStream_t s);
 
+// dsnap = the caller's packed d and *dflag = 0, in one kernel that reads d through the
+// device system table (so a captured graph stays valid across rebinding).  Needs dsnap.
+int mixed_snapshot_d(MixedState& m, hipStream_t s);
+
 const char* mixed_last_error();
 
 }  // namespace bss

@@ -117,12 +117,12 @@ enum bss_option {
                                and solve this and every later right-hand side of the
                                factorisation with it, its backward error measured in
                                FP64; 0 = return BSS_NOT_CONVERGED with the last iterate */
-  BSS_OPT_FACTOR_SWEEP = 11, /* mixed, one right-hand side, n <= 96, no graph, plan-owned
-                               workspace: 1 (default) = bss_factorize also runs the first
+  BSS_OPT_FACTOR_SWEEP = 11, /* mixed, one right-hand side, n <= 96: 1 (default) =
+                               bss_factorize (and bss_batch_factorize) also runs the first
                                solve's level-0 forward sweep on the d it finds (inside the
                                level-0 factor kernel, as bss_factorize_solve does) and keeps
-                               a device copy of d (n nrhs N_use 8 bytes per system, allocated
-                               on first use).  The next bss_solve compares d with that copy
+                               a device copy of d in the plan's workspace (n N_use 8 bytes
+                               per system).  The next bss_solve compares d with that copy
                                on the device and skips its x = 0 pass and level-0 forward
                                sweep if every bit is unchanged, else runs them; the answer is
                                bitwise that of option value 0 either way.  No counterpart in

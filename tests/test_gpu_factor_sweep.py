@@ -11,8 +11,9 @@ import numpy as np
 import pytest
 import torch
 
-from blockstructuredsolvers_amd import (_lib as L, factorize, initialize_batched,
-                                        initialize_batched_group, solve, workloads)
+from blockstructuredsolvers_amd import (_lib as L, factorize, initialize_batch,
+                                        initialize_batched, initialize_batched_group, solve,
+                                        workloads)
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -132,6 +133,72 @@ def test_fp64_separator_path():
 def test_group_bitwise():
     systems = [workloads.spd_system(N, 64, nrhs=1, seed=5400 + N) for N in (3, 200, 777)]
     assert_same(run(systems, True, group=True), run(systems, False, group=True))
+
+
+def test_graph_replays_bitwise():
+    """With hipGraph replay the factorisation's captured sequence holds the d snapshot,
+    the x = 0 pass and the fused level-0 kernel; three factorize/solve steps (the first
+    captures, the others replay) with d restored, then one with a new d, must give the
+    option-off bits every time."""
+    systems = systems_for(64, 900, 2, 5600)
+
+    def steps(on):
+        data = make(systems, graph=True, chain_len=15)
+        L.call("bss_set_option", data._plan, L.BSS_OPT_FACTOR_SWEEP, 1.0 if on else 0.0)
+        d0 = data.d_tensor.clone()
+        outs = []
+        for k in range(4):
+            data.d_tensor.copy_(d0 if k < 3 else d0 * 0.5)
+            factorize(data)
+            outs.append((solve(data), data.iterations))
+            torch.cuda.synchronize()
+            outs.append(data.d_tensor.cpu().numpy().copy())
+        data.close()
+        return outs
+
+    on, off = steps(True), steps(False)
+    for a, b in zip(on, off):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+
+
+def test_uneven_batch_bitwise():
+    """An uneven batch (one group plan per block size, factorisation as one graph):
+    option on against off, bitwise, for two steps with d restored."""
+    Ns, ns = [300, 257, 40, 500, 3], [64, 32, 64, 16, 96]
+    systems = [workloads.spd_system(N, n, nrhs=1, seed=5700 + k)
+               for k, (N, n) in enumerate(zip(Ns, ns))]
+
+    def steps(on):
+        data = initialize_batch(Ns, ns, torch.float64, "cuda", mode="mixed")
+        L.call("bss_batch_set_option", data._batch, L.BSS_OPT_FACTOR_SWEEP, 1.0 if on else 0.0)
+        for k, s in enumerate(systems):
+            data.A_tensors[k][:, :, :s.N].copy_(torch.from_numpy(s.A))
+            if s.N > 1:
+                data.B_tensors[k][:, :, :s.N - 1].copy_(torch.from_numpy(s.B))
+            data.d_tensors[k][:, :, :s.N].copy_(torch.from_numpy(s.d))
+        d0 = [d.clone() for d in data.d_tensors]
+        outs = []
+        for _ in range(2):
+            for d, v in zip(data.d_tensors, d0):
+                d.copy_(v)
+            factorize(data)
+            rc = solve(data)
+            torch.cuda.synchronize()
+            outs.append((rc, [d.cpu().numpy().copy() for d in data.d_tensors],
+                         data.info().tolist()))
+        data.close()
+        return outs
+
+    on, off = steps(True), steps(False)
+    for (ra, xa, ia), (rb, xb, ib) in zip(on, off):
+        assert ra == rb and ia == ib
+        assert all(np.array_equal(a, b) for a, b in zip(xa, xb))
+    for k, s in enumerate(systems):
+        x = np.asfortranarray(on[-1][1][k][:, :, :s.N])
+        assert oracle.backward_error(s.A, s.B, x, s.d).max() <= 1e-12
 
 
 def test_sweep_skipped_when_d_unchanged():
