@@ -539,8 +539,8 @@ def main():
                     help="also run one 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          f"(got {args.gpus})"}))
+        print(json.dumps({"error": "the solver runs on one GPU (no multi-GPU sharding); "
+                          f"--gpus must be 1 (got {args.gpus})"}))
         sys.exit(2)
 
     import torch
