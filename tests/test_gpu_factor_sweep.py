@@ -3,8 +3,8 @@ first solve's x = 0 pass and level-0 forward sweep on the d it finds and keeps a
 d; bss_solve compares d with that copy on the device and skips those two steps only if
 every bit is unchanged.  The answer, info, iteration count and backward errors must be
 bitwise those with the option off, whether d is unchanged, changed in one bit of one
-element, replaced, or solved a second time, for batches, a group and the
-FP64-separator path."""
+element, replaced, or solved a second time, for batches, a group, graph capture and
+replay, an uneven batch across block sizes and the FP64-separator path."""
 import ctypes
 
 import numpy as np
